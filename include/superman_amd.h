@@ -1,0 +1,99 @@
+/*
+ * superman_amd.h -- C ABI of the MI355X exact-permanent library
+ * (libsuperman_amd.so).  Plain pointers and sizes only; every call is
+ * reentrant (no process globals beyond a mutex-guarded per-device cache).
+ *
+ * Each entry point replaces a reference GPU driver (file:line cited per
+ * function).  The reference's drivers return the permanent as a double inside
+ * `Result{time, permanent}` (CommonFiles/Result.h:29-37) and call exit() on a
+ * CUDA error (Algorithms/GPU/GPUHelpers.cuh:77-85); here every call returns a
+ * status code and fills an extended `spm_result`, whose `re` field is exactly
+ * what the reference's Result.permanent carries.  The C++ drop-in layer
+ * (include/superman/*.h) maps these back onto Result / runtime_error.
+ *
+ * Matrices are row-major n x n (a[i*n + j] = a_ij), 1 <= n <= 63
+ * (reference limit: CommonFiles/DecomposePerman.h:185-188).  Complex
+ * matrices are interleaved (re, im) pairs, i.e. std::complex<double>.
+ */
+#ifndef SUPERMAN_AMD_H
+#define SUPERMAN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define SPM_OK            0
+#define SPM_EINVAL       -1   /* bad argument (n out of range, null pointer, ...) */
+#define SPM_EHIP         -2   /* HIP runtime error */
+#define SPM_ENOMEM       -3   /* device allocation failed / over budget */
+#define SPM_ERANGE       -4   /* exact integer result cannot be guaranteed */
+#define SPM_ECOMPILE     -5   /* hiprtc compilation of a generated kernel failed */
+#define SPM_ENOTINT      -6   /* integer path given non-integer entries */
+#define SPM_ECANCELLED   -7   /* request cancelled before it ran */
+#define SPM_EAGAIN       -8   /* admission refused (budget), try later */
+#define SPM_ETIMEDOUT    -9   /* request exceeded its deadline */
+#define SPM_EFAULTINJ   -10   /* fault injected through spm_inject_fault */
+
+/* ---- precision modes (Settings.h:59-65 `Precision`) ---------------------- */
+#define SPM_PREC_PLAIN        0   /* reference "dd": plain FP64 sums          */
+#define SPM_PREC_COMPENSATED  1   /* reference "kahan" (its default, IO.h:178):
+                                     TwoSum-compensated double-double sums   */
+
+/* Extended result.  `re` (and `im` for complex) is the value the reference
+ * returns; hi/lo carry the double-double sum; `abs_sum` = 2 * sum |term|
+ * (compensated modes) so that cond = abs_sum / |value|; `err_est` is an
+ * a-priori bound on the absolute error; `exact_*` hold the exact integer
+ * value (two's complement 128-bit) when `is_exact` is set. */
+typedef struct spm_result {
+    double re, im;
+    double hi_re, lo_re, hi_im, lo_im;
+    double abs_sum;
+    double err_est;
+    int64_t exact_lo, exact_hi;
+    int32_t is_exact;
+    int32_t status;
+    double seconds;       /* host wall time of the call                     */
+    double kernel_ms;     /* device time of the permanent kernels (events)  */
+    uint64_t subsets;     /* 2^(n-1) Gray-code subsets walked               */
+    int32_t n;
+    int32_t launches;
+} spm_result;
+
+const char* spm_version(void);
+const char* spm_strerror(int status);
+/* Number of visible HIP devices (0 if none).  Returns status. */
+int spm_device_count(int* count);
+/* Block until all work queued on `device` has finished. */
+int spm_device_synchronize(int device);
+/* Device description: compute units and engine clock (kHz). */
+int spm_device_info(int device, int* compute_units, int* clock_khz, char* name, int name_len);
+
+/* ---- dense real FP64 ------------------------------------------------------
+ * Replaces gpuDPSingleGPU<double,S> (Algorithms/GPU/GPUKernelWrappers.cu:
+ * 208-268) -> dpSingleGPU::permanentFunction (Algorithms/GPU/Dense/
+ * dpSingleGPU.cuh:44-219) with kernels xRegisterMShared / xRegisterMSharedKahan
+ * (Algorithms/GPU/Dense/DenseKernelDefinitions.cuh:190-405).
+ * `stream` is a hipStream_t (NULL = the device's null stream); the call is
+ * synchronous with respect to it. */
+int spm_dense_f64(const double* a, int n, int precision, int device, void* stream, spm_result* out);
+
+/* Device-resident plans: the matrix is uploaded once, then spm_plan_run
+ * computes with inputs already in HBM (what the benchmark times). */
+typedef struct spm_plan spm_plan;
+int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_plan** plan);
+int spm_plan_run(spm_plan* plan, void* stream, spm_result* out);
+int spm_plan_destroy(spm_plan* plan);
+/* Launch geometry of a plan: grid blocks, block threads, L (Gray steps per
+ * chunk = 2^L), number of wave tasks. */
+int spm_plan_info(const spm_plan* plan, int* grid, int* block, int* log2_chunk, uint64_t* tasks);
+/* Name of the permanent kernel a plan launches (for profiler lookups). */
+const char* spm_plan_kernel_name(const spm_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUPERMAN_AMD_H */

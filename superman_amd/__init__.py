@@ -1,0 +1,11 @@
+"""superman_amd -- MI355X-native exact matrix permanents (Gray-code Ryser).
+
+The compute path is libsuperman_amd.so (hand-written HIP for gfx950 behind a
+C ABI, include/superman_amd.h); this package is a thin Python host layer over
+it.  Importing the package does not load the library; the first call does,
+and raises if the library has not been built (`python -m superman_amd.build`).
+"""
+from .api import (DensePlan, Permanent, dense_f64, device_count, device_info, permanent,  # noqa: F401
+                  synchronize)
+
+__all__ = ["DensePlan", "Permanent", "dense_f64", "device_count", "permanent"]

@@ -1,0 +1,144 @@
+"""Python host API over the C ABI.
+
+    permanent(a, precision="kahan", device=0) -> Permanent
+
+dispatches like the reference's AlgorithmSelector for the GPU modes
+(CommonFiles/AlgorithmSelector.h:48-141): real matrices go to the dense
+Gray-code Ryser kernel.  Precision names follow the reference's
+`calculation_precision` values (IO/IO.h:358-380): "dd" = plain FP64,
+"kahan" = compensated (the reference default).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import SpmResult, check, lib
+
+PRECISIONS = {"dd": _lib.SPM_PREC_PLAIN, "plain": _lib.SPM_PREC_PLAIN,
+              "kahan": _lib.SPM_PREC_COMPENSATED, "compensated": _lib.SPM_PREC_COMPENSATED}
+
+
+@dataclass
+class Permanent:
+    value: complex | float
+    hi: complex | float
+    lo: complex | float
+    abs_sum: float
+    err_est: float
+    exact: int | None
+    seconds: float
+    kernel_ms: float
+    subsets: int
+    n: int
+    launches: int
+
+    @property
+    def cond(self) -> float:
+        v = abs(self.value)
+        return self.abs_sum / v if v else float("inf")
+
+
+def _to_perm(r: SpmResult, cplx: bool) -> Permanent:
+    exact = None
+    if r.is_exact:
+        exact = (int(r.exact_hi) << 64) | (int(r.exact_lo) & ((1 << 64) - 1))
+    if cplx:
+        value, hi, lo = complex(r.re, r.im), complex(r.hi_re, r.hi_im), complex(r.lo_re, r.lo_im)
+    else:
+        value, hi, lo = r.re, r.hi_re, r.lo_re
+    return Permanent(value, hi, lo, r.abs_sum, r.err_est, exact, r.seconds, r.kernel_ms,
+                     int(r.subsets), int(r.n), int(r.launches))
+
+
+def _prec(p) -> int:
+    if isinstance(p, int):
+        return p
+    try:
+        return PRECISIONS[p]
+    except KeyError:
+        raise ValueError(f"unknown precision {p!r}") from None
+
+
+def _square(a, dtype) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise ValueError("square matrix expected")
+    return a
+
+
+def dense_f64(a, precision="kahan", device: int = 0, stream: int | None = None) -> Permanent:
+    a = _square(a, np.float64)
+    r = SpmResult()
+    st = lib().spm_dense_f64(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0],
+                             _prec(precision), device, stream, ctypes.byref(r))
+    check(st, "spm_dense_f64")
+    return _to_perm(r, False)
+
+
+def permanent(a, precision="kahan", device: int = 0) -> Permanent:
+    """Permanent of a square matrix on the GPU."""
+    a = np.asarray(a)
+    if np.iscomplexobj(a):
+        raise NotImplementedError("complex path")
+    return dense_f64(a, precision, device)
+
+
+class DensePlan:
+    """A dense FP64 permanent with the matrix resident on the device."""
+
+    def __init__(self, a, precision="kahan", device: int = 0):
+        a = _square(a, np.float64)
+        self._h = ctypes.c_void_p()
+        check(lib().spm_plan_dense_f64(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0],
+                                       _prec(precision), device, ctypes.byref(self._h)), "spm_plan_dense_f64")
+        self.n = a.shape[0]
+
+    def run(self, stream: int | None = None) -> Permanent:
+        r = SpmResult()
+        check(lib().spm_plan_run(self._h, stream, ctypes.byref(r)), "spm_plan_run")
+        return _to_perm(r, False)
+
+    def info(self) -> dict:
+        g, b, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        t = ctypes.c_uint64()
+        check(lib().spm_plan_info(self._h, ctypes.byref(g), ctypes.byref(b), ctypes.byref(l), ctypes.byref(t)))
+        return {"grid": g.value, "block": b.value, "log2_chunk": l.value, "tasks": t.value,
+                "kernel": lib().spm_plan_kernel_name(self._h).decode()}
+
+    def close(self):
+        if self._h:
+            lib().spm_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def device_count() -> int:
+    c = ctypes.c_int()
+    check(lib().spm_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def synchronize(device: int = 0) -> None:
+    check(lib().spm_device_synchronize(device), "spm_device_synchronize")
+
+
+def device_info(device: int = 0) -> dict:
+    cu, clk = ctypes.c_int(), ctypes.c_int()
+    name = ctypes.create_string_buffer(256)
+    check(lib().spm_device_info(device, ctypes.byref(cu), ctypes.byref(clk), name, 256), "spm_device_info")
+    return {"compute_units": cu.value, "clock_khz": clk.value, "name": name.value.decode()}

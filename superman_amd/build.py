@@ -1,0 +1,86 @@
+"""Builds libsuperman_amd.so in-tree with hipcc for gfx950.
+
+    python -m superman_amd.build [-j JOBS] [--verbose]
+
+Every HIP translation unit is compiled with
+    -Xclang -target-feature -Xclang -packed-fp32-ops
+(no packed FP32 instructions are generated), objects go to
+superman_amd/_build/, and the shared library is linked next to this file.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shlex
+import subprocess
+import sys
+import time
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libsuperman_amd.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+          "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+# (source, extra flags).  .hip files carry device code; .cpp files are host
+# code compiled by the same driver (HIP runtime API).
+SOURCES = [
+    ("dense_f64.hip", []),
+    ("capi.cpp", []),
+]
+LINK_LIBS = ["-lhiprtc", "-lpthread"]
+
+
+def _compile(src: str, flags: list[str], verbose: bool) -> tuple[str, float, str]:
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + COMMON + DEVICE + flags + ["-c", path, "-o", obj]
+    if verbose:
+        print(" ".join(shlex.quote(c) for c in cmd), flush=True)
+    t = time.time()
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError(f"compile failed: {src}\n{out.stdout}\n{out.stderr}")
+    return obj, time.time() - t, out.stderr
+
+
+def build(jobs: int = 8, verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    t0 = time.time()
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        futs = [ex.submit(_compile, s, f, verbose) for s, f in SOURCES]
+        for fu in futs:
+            obj, dt, _ = fu.result()
+            objs.append(obj)
+            if verbose:
+                print(f"  {os.path.basename(obj)}: {dt:.1f}s", flush=True)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + LINK_LIBS
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError(f"link failed\n{out.stdout}\n{out.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB} in {time.time() - t0:.1f}s", flush=True)
+    return LIB
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", "--jobs", type=int, default=min(8, os.cpu_count() or 1))
+    ap.add_argument("--verbose", "-v", action="store_true")
+    a = ap.parse_args(argv)
+    build(a.jobs, a.verbose)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

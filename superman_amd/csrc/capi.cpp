@@ -1,0 +1,367 @@
+// capi.cpp -- the C ABI (include/superman_amd.h) for the dense FP64 path:
+// host-side preparation, launch geometry, device buffers, launches, results.
+//
+// Host preparation is the reference's own (dpSingleGPU.cuh:62-82): x(empty)
+// with the row sum taken left to right in FP64, and the column-major copy of
+// the matrix.  Everything per-subset runs on the GPU.
+#include "superman_amd.h"
+
+#include "spm_host.h"
+#include "spm_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace spm {
+
+// ---------------------------------------------------------------------------
+// Device properties and occupancy, cached per (device, kernel).
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_mu;
+struct OccEntry {
+    int device;
+    const void* kern;
+    int block;
+    int bpc;
+};
+std::vector<OccEntry> g_occ;
+int g_cus[64] = {0};
+}  // namespace
+
+int device_cus(int device, int* cus)
+{
+    if (device < 0 || device >= 64) return SPM_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (g_cus[device] > 0) {
+            *cus = g_cus[device];
+            return SPM_OK;
+        }
+    }
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || v <= 0)
+        return SPM_EHIP;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cus[device] = v;
+    *cus = v;
+    return SPM_OK;
+}
+
+int blocks_per_cu(int device, const void* kern, int block, int* bpc)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (const auto& e : g_occ)
+            if (e.device == device && e.kern == kern && e.block == block) {
+                *bpc = e.bpc;
+                return SPM_OK;
+            }
+    }
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kern, block, 0) != hipSuccess || v <= 0) return SPM_EHIP;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_occ.push_back({device, kern, block, v});
+    *bpc = v;
+    return SPM_OK;
+}
+
+// Scoped hipSetDevice that restores the caller's current device.
+DeviceGuard::DeviceGuard(int device)
+{
+    if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
+    ok_ = hipSetDevice(device) == hipSuccess;
+}
+DeviceGuard::~DeviceGuard()
+{
+    if (prev_ >= 0) (void)hipSetDevice(prev_);
+}
+
+double now_seconds()
+{
+    using namespace std::chrono;
+    return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Chunk size 2^L per lane.  Small enough that every wave slot gets work
+// (tasks >= resident waves), large enough that rebuilding x at a chunk start
+// (~n (n-L) FMAs) costs <~1.5% of the chunk's 2^L (2n-1) ops.
+void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks)
+{
+    const int B = n - 1;
+    const int lo = unroll_bits > 0 ? unroll_bits + 1 : 0;
+    int La = lo;
+    while (La < B && std::ldexp(1.0, La) * (2.0 * n - 1.0) < 64.0 * n * (double)(n - La)) ++La;
+    int lw = 0;
+    while ((int64_t(1) << lw) < std::max<int64_t>(resident_waves, 1)) ++lw;
+    const int Lp = B - 6 - lw;
+    int l = std::max(lo, std::min(La, Lp));
+    l = std::min(l, B);
+    *L = l;
+    *chunks = uint64_t(1) << (B - l);
+    *tasks = (*chunks + kWaveLanes - 1) / kWaveLanes;
+}
+
+}  // namespace spm
+
+using namespace spm;
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+struct spm_plan {
+    int kind = 0;  // 1 = dense f64
+    int n = 0;
+    int precision = 0;
+    int device = 0;
+    int grid = 0;
+    int L = 0;
+    uint64_t chunks = 0, tasks = 0;
+    int slices = 1;
+    const void* kern = nullptr;
+    double* d_M = nullptr;
+    double* d_x0 = nullptr;
+    double* d_partial = nullptr;
+    double* d_out = nullptr;
+    double* h_out = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string kname;
+};
+
+static void plan_free(spm_plan* p)
+{
+    if (!p) return;
+    DeviceGuard g(p->device);
+    if (p->d_M) (void)hipFree(p->d_M);
+    if (p->d_x0) (void)hipFree(p->d_x0);
+    if (p->d_partial) (void)hipFree(p->d_partial);
+    if (p->d_out) (void)hipFree(p->d_out);
+    if (p->h_out) (void)hipHostFree(p->h_out);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    delete p;
+}
+
+#define SPM_TRY(expr)                                  \
+    do {                                               \
+        if ((expr) != hipSuccess) {                    \
+            plan_free(p);                              \
+            return SPM_EHIP;                           \
+        }                                              \
+    } while (0)
+
+extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_plan** out)
+{
+    if (!out) return SPM_EINVAL;
+    *out = nullptr;
+    if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
+    if (precision != SPM_PREC_PLAIN && precision != SPM_PREC_COMPENSATED) return SPM_EINVAL;
+    for (int e = 0; e < n * n; ++e)
+        if (!std::isfinite(a[e])) return SPM_EINVAL;
+    int cus = 0;
+    int st = device_cus(device, &cus);
+    if (st != SPM_OK) return st;
+    DeviceGuard guard(device);
+    if (!guard.ok()) return SPM_EHIP;
+
+    spm_plan* p = new spm_plan;
+    p->kind = 1;
+    p->n = n;
+    p->precision = precision;
+    p->device = device;
+    p->kern = dense_f64_kernel_ptr(n, precision == SPM_PREC_COMPENSATED);
+    int bpc = 0;
+    st = blocks_per_cu(device, p->kern, kDenseBlock, &bpc);
+    if (st != SPM_OK) {
+        plan_free(p);
+        return st;
+    }
+    const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
+    gray_geometry(n, dense_f64_unroll_bits(n), waves, &p->L, &p->chunks, &p->tasks);
+    const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
+    p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
+    // Split very long runs (~> 4 s of VALU work at 64 lanes x 256 CUs x 2.4 GHz)
+    // into several launches that accumulate into the same per-block records.
+    const double est_s = std::ldexp(1.0, n - 1) * (2.0 * n) / 3.9e13;
+    p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
+    char kn[96];
+    std::snprintf(kn, sizeof kn, "dense_f64_kernel<%d, %d, %s>", n, dense_f64_unroll_bits(n),
+                  precision == SPM_PREC_COMPENSATED ? "true" : "false");
+    p->kname = kn;
+
+    // Host preparation (dpSingleGPU.cuh:62-82).
+    std::vector<double> colmaj((size_t)n * n), x0(n);
+    for (int i = 0; i < n; ++i) {
+        double rowSum = 0;
+        for (int j = 0; j < n; ++j) rowSum += a[i * n + j];
+        x0[i] = a[i * n + (n - 1)] - (rowSum / 2);
+        for (int j = 0; j < n; ++j) colmaj[(size_t)j * n + i] = a[i * n + j];
+    }
+    SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * n * n));
+    SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * n));
+    SPM_TRY(hipMalloc(&p->d_partial, sizeof(double) * 4 * p->grid));
+    SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 4));
+    SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 4, hipHostMallocDefault));
+    SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+    SPM_TRY(hipMemcpy(p->d_x0, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    SPM_TRY(hipEventCreate(&p->ev0));
+    SPM_TRY(hipEventCreate(&p->ev1));
+    *out = p;
+    return SPM_OK;
+}
+
+#undef SPM_TRY
+
+static int dense_run(spm_plan* p, hipStream_t st, spm_result* r)
+{
+    const double t0 = now_seconds();
+    DeviceGuard guard(p->device);
+    if (!guard.ok()) return SPM_EHIP;
+    if (hipEventRecord(p->ev0, st) != hipSuccess) return SPM_EHIP;
+    const uint64_t per = (p->tasks + p->slices - 1) / p->slices;
+    int launches = 0;
+    for (int s = 0; s < p->slices; ++s) {
+        uint64_t tb = per * s, te = std::min<uint64_t>(p->tasks, per * (s + 1));
+        if (tb >= te) break;
+        int acc = s > 0 ? 1 : 0;
+        void* args[] = {&p->d_M, &p->d_x0, &p->d_partial, &tb, &te, &p->chunks, &p->L, &acc};
+        if (hipLaunchKernel(p->kern, dim3(p->grid), dim3(kDenseBlock), args, 0, st) != hipSuccess) return SPM_EHIP;
+        ++launches;
+    }
+    {
+        int nrec = p->grid, stride = 4, comps = 1;
+        void* args[] = {&p->d_partial, &nrec, &stride, &comps, &p->d_out};
+        if (hipLaunchKernel(reduce_partials_kernel_ptr(), dim3(1), dim3(256), args, 0, st) != hipSuccess)
+            return SPM_EHIP;
+    }
+    if (hipEventRecord(p->ev1, st) != hipSuccess) return SPM_EHIP;
+    if (hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * 3, hipMemcpyDeviceToHost, st) != hipSuccess)
+        return SPM_EHIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return SPM_EHIP;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p->ev0, p->ev1) != hipSuccess) ms = -1.f;
+
+    std::memset(r, 0, sizeof *r);
+    const int n = p->n;
+    const double f = (double)(4 * (n % 2) - 2);  // 2 (-1)^(n-1), dpSingleGPU.cuh:218
+    const double hi = p->h_out[0], lo = p->h_out[1];
+    r->hi_re = f * hi;
+    r->lo_re = f * lo;
+    r->re = f * (hi + lo);
+    r->im = 0.0;
+    r->abs_sum = 2.0 * p->h_out[2];
+    r->err_est = p->precision == SPM_PREC_COMPENSATED ? dense_err_est(n, r->abs_sum) : -1.0;
+    r->status = SPM_OK;
+    r->kernel_ms = ms;
+    r->subsets = uint64_t(1) << (n - 1);
+    r->n = n;
+    r->launches = launches;
+    r->seconds = now_seconds() - t0;
+    return SPM_OK;
+}
+
+extern "C" int spm_plan_run(spm_plan* p, void* stream, spm_result* out)
+{
+    if (!p || !out) return SPM_EINVAL;
+    if (p->kind == 1) return dense_run(p, (hipStream_t)stream, out);
+    return SPM_EINVAL;
+}
+
+extern "C" int spm_plan_destroy(spm_plan* p)
+{
+    plan_free(p);
+    return SPM_OK;
+}
+
+extern "C" int spm_plan_info(const spm_plan* p, int* grid, int* block, int* log2_chunk, uint64_t* tasks)
+{
+    if (!p) return SPM_EINVAL;
+    if (grid) *grid = p->grid;
+    if (block) *block = kDenseBlock;
+    if (log2_chunk) *log2_chunk = p->L;
+    if (tasks) *tasks = p->tasks;
+    return SPM_OK;
+}
+
+extern "C" const char* spm_plan_kernel_name(const spm_plan* p) { return p ? p->kname.c_str() : ""; }
+
+extern "C" int spm_dense_f64(const double* a, int n, int precision, int device, void* stream, spm_result* out)
+{
+    if (!out) return SPM_EINVAL;
+    const double t0 = now_seconds();
+    spm_plan* p = nullptr;
+    int st = spm_plan_dense_f64(a, n, precision, device, &p);
+    if (st != SPM_OK) {
+        std::memset(out, 0, sizeof *out);
+        out->status = st;
+        return st;
+    }
+    st = dense_run(p, (hipStream_t)stream, out);
+    plan_free(p);
+    if (st == SPM_OK) out->seconds = now_seconds() - t0;
+    else out->status = st;
+    return st;
+}
+
+// ---------------------------------------------------------------------------
+// Misc
+// ---------------------------------------------------------------------------
+extern "C" const char* spm_version(void) { return "superman_amd 0.1 (gfx950)"; }
+
+extern "C" const char* spm_strerror(int s)
+{
+    switch (s) {
+    case SPM_OK: return "ok";
+    case SPM_EINVAL: return "invalid argument";
+    case SPM_EHIP: return "HIP runtime error";
+    case SPM_ENOMEM: return "out of device memory or over budget";
+    case SPM_ERANGE: return "exact result not representable (bound exceeded)";
+    case SPM_ECOMPILE: return "generated kernel failed to compile";
+    case SPM_ENOTINT: return "matrix has non-integer entries";
+    case SPM_ECANCELLED: return "cancelled";
+    case SPM_EAGAIN: return "admission refused, retry later";
+    case SPM_ETIMEDOUT: return "deadline exceeded";
+    case SPM_EFAULTINJ: return "injected fault";
+    default: return "unknown status";
+    }
+}
+
+extern "C" int spm_device_count(int* count)
+{
+    if (!count) return SPM_EINVAL;
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *count = 0;
+        return e == hipErrorNoDevice ? SPM_OK : SPM_EHIP;
+    }
+    *count = c;
+    return SPM_OK;
+}
+
+extern "C" int spm_device_synchronize(int device)
+{
+    DeviceGuard g(device);
+    if (!g.ok()) return SPM_EHIP;
+    return hipDeviceSynchronize() == hipSuccess ? SPM_OK : SPM_EHIP;
+}
+
+extern "C" int spm_device_info(int device, int* compute_units, int* clock_khz, char* name, int name_len)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return SPM_EHIP;
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    return SPM_OK;
+}

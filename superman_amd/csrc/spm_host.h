@@ -1,0 +1,40 @@
+// spm_host.h -- host-side helpers shared by the drivers behind the C ABI.
+#pragma once
+
+#include <stdint.h>
+
+namespace spm {
+
+constexpr int kWaveLanes = 64;
+
+int device_cus(int device, int* cus);
+int blocks_per_cu(int device, const void* kern, int block, int* bpc);
+double now_seconds();
+
+// Gray-walk geometry shared by the dense kernels: 2^L steps per lane chunk,
+// 2^(n-1-L) chunks, grouped 64 per wave task.
+void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks);
+
+// A-priori absolute error estimate for a compensated FP64 real run:
+// each term carries ~(n-1) product roundings plus the rounding of its n
+// incrementally-updated factors; abs_sum = 2 sum |term|.
+inline double dense_err_est(int n, double abs_sum)
+{
+    const double u = 1.1102230246251565e-16;  // 2^-53
+    return 2.0 * n * u * abs_sum;
+}
+
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int device);
+    ~DeviceGuard();
+    bool ok() const { return ok_; }
+    DeviceGuard(const DeviceGuard&) = delete;
+    DeviceGuard& operator=(const DeviceGuard&) = delete;
+
+private:
+    int prev_ = -1;
+    bool ok_ = false;
+};
+
+}  // namespace spm

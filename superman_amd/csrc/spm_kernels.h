@@ -1,0 +1,23 @@
+// spm_kernels.h -- host-visible handles to the device kernels.
+//
+// Each kernel family exposes a lookup returning the __global__ function's
+// host stub address, launched with hipLaunchKernel by the drivers.
+#pragma once
+
+#include <stdint.h>
+
+namespace spm {
+
+constexpr int kMaxN = 63;          // 64-bit Gray index (reference README.md:75)
+constexpr int kDenseBlock = 256;   // threads per block of the dense kernels
+
+// dense_f64.hip
+//   args: (const double* M_colmajor, const double* x0, double* partial,
+//          uint64_t task_begin, uint64_t task_end, uint64_t num_chunks, int L,
+//          int accumulate)
+const void* dense_f64_kernel_ptr(int n, bool compensated);
+int dense_f64_unroll_bits(int n);
+//   args: (const double* partial, int nrec, int stride, int comps, double* out)
+const void* reduce_partials_kernel_ptr();
+
+}  // namespace spm
