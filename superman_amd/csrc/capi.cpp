@@ -97,7 +97,10 @@ double now_seconds()
 void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks)
 {
     const int B = n - 1;
-    const int lo = unroll_bits > 0 ? unroll_bits + 1 : 0;
+    // L >= 1 whenever there is a column to flip: the kernel takes the term
+    // sign (-1)^k from the low L bits, so a chunk must start at an even k;
+    // with unrolling, L > U so that block m's bit U exists inside the chunk.
+    const int lo = unroll_bits > 0 ? unroll_bits + 1 : (B >= 1 ? 1 : 0);
     int La = lo;
     while (La < B && std::ldexp(1.0, La) * (2.0 * n - 1.0) < 64.0 * n * (double)(n - La)) ++La;
     int lw = 0;
@@ -344,6 +347,13 @@ extern "C" int spm_device_count(int* count)
         return e == hipErrorNoDevice ? SPM_OK : SPM_EHIP;
     }
     *count = c;
+    return SPM_OK;
+}
+
+extern "C" int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk, uint64_t* chunks, uint64_t* tasks)
+{
+    if (n < 1 || n > kMaxN || !log2_chunk || !chunks || !tasks) return SPM_EINVAL;
+    gray_geometry(n, dense_f64_unroll_bits(n), resident_waves, log2_chunk, chunks, tasks);
     return SPM_OK;
 }
 
