@@ -94,6 +94,10 @@ def main(argv=None):
     S.synchronize(0)
     dt = time.perf_counter() - t0
     plan.close()
+    # the other accumulation mode, once, outside the timed region
+    other = "dd" if args.precision == "kahan" else "kahan"
+    with S.DensePlan(a, other) as p2:
+        r_other = p2.run()
 
     ms_per_step = dt * 1e3 / args.steps
     flops = 2.0 * n * float(1 << (n - 1))  # algorithmic 2n flops per subset
@@ -120,7 +124,8 @@ def main(argv=None):
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel": info["kernel"], "kernel_ms": k_ms,
                      "flops_per_launch": flops, "flop_convention": "2n per subset (add, mul, accumulate; FMA = 2)"},
-        "result": {"value": r.value, "err_est": r.err_est, "cond": r.cond},
+        "result": {"value": r.value, "err_est": r.err_est, "cond": r.cond,
+                   f"{other}_value": r_other.value, f"{other}_kernel_ms": r_other.kernel_ms},
         "geometry": info,
         "device": dev,
     }

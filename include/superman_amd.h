@@ -92,8 +92,10 @@ int spm_plan_info(const spm_plan* plan, int* grid, int* block, int* log2_chunk, 
 /* The Gray-walk geometry the dense kernels use for order n when
  * `resident_waves` wavefronts fit on the device: each lane walks 2^log2_chunk
  * consecutive Gray indices; chunks = 2^(n-1-log2_chunk); tasks = chunks / 64
- * rounded up.  Pure host computation. */
-int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk, uint64_t* chunks, uint64_t* tasks);
+ * rounded up; the kernel unrolls blocks of 2^unroll_bits steps (may be NULL).
+ * Pure host computation. */
+int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk, uint64_t* chunks, uint64_t* tasks,
+                      int* unroll_bits);
 /* Name of the permanent kernel a plan launches (for profiler lookups). */
 const char* spm_plan_kernel_name(const spm_plan* plan);
 

@@ -52,34 +52,45 @@ def _compile(src: str, flags: list[str], verbose: bool) -> tuple[str, float, str
     return obj, time.time() - t, out.stderr
 
 
-def build(jobs: int = 8, verbose: bool = False) -> str:
+def build(jobs: int = 8, verbose: bool = False, defines=(), out: str | None = None,
+          build_dir: str | None = None) -> str:
+    """Compile and link.  `defines` (NAME=VALUE) and `out`/`build_dir` let a
+    tuning sweep build variants of the library away from the package."""
+    global BUILD
+    lib_path = out or LIB
+    if build_dir:
+        BUILD = build_dir
     os.makedirs(BUILD, exist_ok=True)
+    extra = ["-D" + d for d in defines]
     t0 = time.time()
     objs = []
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        futs = [ex.submit(_compile, s, f, verbose) for s, f in SOURCES]
+        futs = [ex.submit(_compile, s, f + extra, verbose) for s, f in SOURCES]
         for fu in futs:
             obj, dt, _ = fu.result()
             objs.append(obj)
             if verbose:
                 print(f"  {os.path.basename(obj)}: {dt:.1f}s", flush=True)
-    tmp = LIB + ".tmp"
+    tmp = lib_path + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + LINK_LIBS
     out = subprocess.run(cmd, capture_output=True, text=True)
     if out.returncode != 0:
         raise RuntimeError(f"link failed\n{out.stdout}\n{out.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib_path)
     if verbose:
-        print(f"built {LIB} in {time.time() - t0:.1f}s", flush=True)
-    return LIB
+        print(f"built {lib_path} in {time.time() - t0:.1f}s", flush=True)
+    return lib_path
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", "--jobs", type=int, default=min(8, os.cpu_count() or 1))
     ap.add_argument("--verbose", "-v", action="store_true")
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="NAME=VALUE for a tuning variant")
+    ap.add_argument("--out", help="library path (default: in the package)")
+    ap.add_argument("--build-dir", help="object directory (default: superman_amd/_build)")
     a = ap.parse_args(argv)
-    build(a.jobs, a.verbose)
+    build(a.jobs, a.verbose, a.defines, a.out, a.build_dir)
 
 
 if __name__ == "__main__":

@@ -20,6 +20,10 @@
 #include <string>
 #include <vector>
 
+#ifndef SPM_CHUNK_AMORTIZE
+#define SPM_CHUNK_AMORTIZE 1024  // chunk work >= this x its x-rebuild work
+#endif
+
 namespace spm {
 
 // ---------------------------------------------------------------------------
@@ -93,7 +97,8 @@ double now_seconds()
 
 // Chunk size 2^L per lane.  Small enough that every wave slot gets work
 // (tasks >= resident waves), large enough that rebuilding x at a chunk start
-// (~n (n-L) FMAs) costs <~1.5% of the chunk's 2^L (2n-1) ops.
+// (~n (n-L) FMAs plus the task bookkeeping) is <~0.1% of the chunk's
+// 2^L (2n-1) ops (measured at n=40: 2^10-step chunks cost 3% more time).
 void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks)
 {
     const int B = n - 1;
@@ -102,7 +107,7 @@ void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint6
     // with unrolling, L > U so that block m's bit U exists inside the chunk.
     const int lo = unroll_bits > 0 ? unroll_bits + 1 : (B >= 1 ? 1 : 0);
     int La = lo;
-    while (La < B && std::ldexp(1.0, La) * (2.0 * n - 1.0) < 64.0 * n * (double)(n - La)) ++La;
+    while (La < B && std::ldexp(1.0, La) * (2.0 * n - 1.0) < double(SPM_CHUNK_AMORTIZE) * n * (double)(n - La)) ++La;
     int lw = 0;
     while ((int64_t(1) << lw) < std::max<int64_t>(resident_waves, 1)) ++lw;
     const int Lp = B - 6 - lw;
@@ -181,6 +186,10 @@ extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int dev
     p->precision = precision;
     p->device = device;
     p->kern = dense_f64_kernel_ptr(n, precision == SPM_PREC_COMPENSATED);
+    if (!p->kern) {
+        plan_free(p);
+        return SPM_EINVAL;
+    }
     int bpc = 0;
     st = blocks_per_cu(device, p->kern, kDenseBlock, &bpc);
     if (st != SPM_OK) {
@@ -200,20 +209,25 @@ extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int dev
                   precision == SPM_PREC_COMPENSATED ? "true" : "false");
     p->kname = kn;
 
-    // Host preparation (dpSingleGPU.cuh:62-82).
-    std::vector<double> colmaj((size_t)n * n), x0(n);
+    // Host preparation (dpSingleGPU.cuh:62-82), plus a negated copy of the
+    // column-major matrix: the kernel picks the plane by flip direction.
+    const size_t nn = (size_t)n * n;
+    std::vector<double> colmaj(2 * nn), x0(n);
     for (int i = 0; i < n; ++i) {
         double rowSum = 0;
         for (int j = 0; j < n; ++j) rowSum += a[i * n + j];
         x0[i] = a[i * n + (n - 1)] - (rowSum / 2);
-        for (int j = 0; j < n; ++j) colmaj[(size_t)j * n + i] = a[i * n + j];
+        for (int j = 0; j < n; ++j) {
+            colmaj[(size_t)j * n + i] = a[i * n + j];
+            colmaj[nn + (size_t)j * n + i] = -a[i * n + j];
+        }
     }
-    SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * n * n));
+    SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * nn));
     SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * n));
     SPM_TRY(hipMalloc(&p->d_partial, sizeof(double) * 4 * p->grid));
     SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 4));
     SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 4, hipHostMallocDefault));
-    SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+    SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice));
     SPM_TRY(hipMemcpy(p->d_x0, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     SPM_TRY(hipEventCreate(&p->ev0));
     SPM_TRY(hipEventCreate(&p->ev1));
@@ -350,10 +364,12 @@ extern "C" int spm_device_count(int* count)
     return SPM_OK;
 }
 
-extern "C" int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk, uint64_t* chunks, uint64_t* tasks)
+extern "C" int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk, uint64_t* chunks, uint64_t* tasks,
+                                 int* unroll_bits)
 {
     if (n < 1 || n > kMaxN || !log2_chunk || !chunks || !tasks) return SPM_EINVAL;
     gray_geometry(n, dense_f64_unroll_bits(n), resident_waves, log2_chunk, chunks, tasks);
+    if (unroll_bits) *unroll_bits = dense_f64_unroll_bits(n);
     return SPM_OK;
 }
 

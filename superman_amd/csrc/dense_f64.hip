@@ -40,7 +40,32 @@
 
 #include <type_traits>
 
+// Tuning knobs (build-time).  SPM_DENSE_WAVES_PER_EU caps VGPRs so that this
+// many waves fit per SIMD (0 = compiler's choice); SPM_DENSE_UNROLL is U, the
+// log2 of the steps per unrolled block.
+#ifndef SPM_DENSE_WAVES_PER_EU
+#define SPM_DENSE_WAVES_PER_EU 0
+#endif
+#ifndef SPM_DENSE_UNROLL
+#define SPM_DENSE_UNROLL 1
+#endif
+// Steps whose terms are summed in plain FP64 before each compensated fold.
+#ifndef SPM_DENSE_FOLD_STEPS
+#define SPM_DENSE_FOLD_STEPS 16
+#endif
+// SPM_DENSE_ONLY_N (sweeps only): instantiate a single order.
+#ifndef SPM_DENSE_ONLY_N
+#define SPM_DENSE_ONLY_N 0
+#endif
+#if SPM_DENSE_WAVES_PER_EU > 0
+#define SPM_DENSE_ATTR __attribute__((amdgpu_waves_per_eu(SPM_DENSE_WAVES_PER_EU)))
+#else
+#define SPM_DENSE_ATTR
+#endif
+
 namespace spm {
+
+constexpr uint32_t kFoldSteps = SPM_DENSE_FOLD_STEPS;
 
 template <int J, int END, class F>
 __device__ __forceinline__ void static_for(F&& f)
@@ -115,9 +140,78 @@ __device__ __forceinline__ void flip_const(double (&x)[N], const double* __restr
     for (int r = 0; r < N; ++r) x[r] = NEG ? x[r] - col[r] : x[r] + col[r];
 }
 
+// A matrix column held in SGPRs (wave-uniform values from scalar loads).
+// The pipelined walk keeps a prefetched column live across a step's
+// multiplies; a full column needs 2N SGPRs, which fits the 102-SGPR file up
+// to N = 40.  Larger orders prefetch a 32-value head and load the tail when
+// the column is applied.
+__device__ __forceinline__ void step_fence()
+{
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+#ifndef SPM_DENSE_PREFETCH_MAX
+#define SPM_DENSE_PREFETCH_MAX 40
+#endif
+template <int N>
+constexpr int kColumnHead = N <= SPM_DENSE_PREFETCH_MAX ? N : 32;
+
+// Unroll depth per order: the pipelined 2-step blocks up to the order whose
+// prefetched column still fits the SGPR file; single steps beyond that and
+// for the tiny orders.
+template <int N>
+constexpr int kDenseUnroll = (N >= SPM_DENSE_UNROLL + 2 && N <= SPM_DENSE_PREFETCH_MAX) ? SPM_DENSE_UNROLL : 0;
+
+template <int N, bool TAIL = (kColumnHead<N> < N)>
+struct Column {
+    double v[kColumnHead<N>];
+};
+template <int N>
+struct Column<N, true> {
+    double v[kColumnHead<N>];
+    const double* p;  // column base, for the tail loads
+};
+
+template <int N>
+__device__ __forceinline__ void load_column(Column<N>& c, const double* p)
+{
+    const const_f64* __restrict__ q = step_column(p);
+#pragma unroll
+    for (int r = 0; r < kColumnHead<N>; ++r) c.v[r] = q[r];
+    if constexpr (kColumnHead<N> < N) c.p = p;
+}
+
+template <int N>
+__device__ __forceinline__ void add_column(double (&x)[N], const Column<N>& c)
+{
+    constexpr int H = kColumnHead<N>;
+#pragma unroll
+    for (int r = 0; r < H; ++r) x[r] += c.v[r];
+    if constexpr (H < N) {
+        step_fence();  // the head's SGPRs are free before the tail loads
+        const const_f64* __restrict__ q = step_column(c.p);
+#pragma unroll
+        for (int r = H; r < N; ++r) x[r] += q[r];
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void fma_column(double (&x)[N], const Column<N>& c, double s)
+{
+    constexpr int H = kColumnHead<N>;
+#pragma unroll
+    for (int r = 0; r < H; ++r) x[r] = __builtin_fma(s, c.v[r], x[r]);
+    if constexpr (H < N) {
+        step_fence();
+        const const_f64* __restrict__ q = step_column(c.p);
+#pragma unroll
+        for (int r = H; r < N; ++r) x[r] = __builtin_fma(s, q[r], x[r]);
+    }
+}
+
 template <int N, int U, bool COMP>
-__global__ void __launch_bounds__(kDenseBlock)
-dense_f64_kernel(const double* __restrict__ M,   // column-major: column c at M + c*N
+__global__ void __launch_bounds__(kDenseBlock) SPM_DENSE_ATTR
+dense_f64_kernel(const double* __restrict__ M,   // [2][N][N]: +A then -A, column-major
                  const double* __restrict__ x0,  // x(empty set), N values
                  double* __restrict__ partial,   // [gridDim.x][4]: hi, lo, abs, 0
                  uint64_t task_begin, uint64_t task_end, uint64_t num_chunks, int L,
@@ -135,14 +229,19 @@ dense_f64_kernel(const double* __restrict__ M,   // column-major: column c at M 
         const bool active = h < num_chunks;  // lanes past the end walk a dummy chunk
 
         // x at the chunk start G(h 2^L): columns >= L-1 may be set.
+        // (Loads go through step_column too: kept as loop invariants, x0 alone
+        // would pin 2N SGPRs for the whole task loop.)
         double x[N];
+        {
+            const const_f64* __restrict__ xs = step_column(x0);
 #pragma unroll
-        for (int r = 0; r < N; ++r) x[r] = x0[r];
+            for (int r = 0; r < N; ++r) x[r] = xs[r];
+        }
         const uint64_t k0 = h << L;
         const uint64_t g0 = k0 ^ (k0 >> 1);
         for (int j = (L > 0 ? L - 1 : 0); j < N - 1; ++j) {
             const double b = ((g0 >> j) & 1u) ? 1.0 : 0.0;
-            const double* __restrict__ col = M + j * N;
+            const const_f64* __restrict__ col = step_column(M + j * N);
 #pragma unroll
             for (int r = 0; r < N; ++r) x[r] = __builtin_fma(b, col[r], x[r]);
         }
@@ -150,38 +249,110 @@ dense_f64_kernel(const double* __restrict__ M,   // column-major: column c at M 
         const double lane_sign = (h & 1u) ? -1.0 : 1.0;  // direction of column L-1
         const uint32_t nblk = 1u << (L - U);
         double t_hi = 0.0, t_lo = 0.0, t_abs = 0.0;
-        for (uint32_t m = 0; m < nblk; ++m) {
-            double blk = 0.0, blk_abs = 0.0;
-            // step j = 0: runtime column ctz(m) + U (none for the chunk start)
-            if (m != 0) {
-                const int tz = __builtin_ctz(m);
-                const int c = tz + U;
-                const double s = (c + 1 < L) ? (((m >> (tz + 1)) & 1u) ? -1.0 : 1.0) : lane_sign;
-                flip_signed<N>(x, M + c * N, s);
-            }
-            add_term<N, COMP>(x, U == 0 && (m & 1u), blk, blk_abs);
-            if constexpr (U > 0) {
-                // bit U of the step index.  Held in a VGPR: a VOP3 op takes one
-                // SGPR source, and that slot belongs to the column value.
-                double s_mid = (m & 1u) ? -1.0 : 1.0;
-                asm("" : "+v"(s_mid));
-                static_for<1, (1 << U)>([&](auto J) {
-                    constexpr int j = decltype(J)::value;
-                    constexpr int c = __builtin_ctz(j);
-                    if constexpr (c + 1 < U) {
-                        flip_const<N, ((j >> (c + 1)) & 1) != 0>(x, M + c * N);
-                    } else {
-                        flip_signed<N>(x, M + c * N, s_mid);
-                    }
-                    add_term<N, COMP>(x, (j & 1) != 0, blk, blk_abs);
-                });
-            }
+        double blk = 0.0, blk_abs = 0.0;
+        // Block sums accumulate in plain FP64 for kFoldSteps steps, then fold
+        // into the compensated per-lane sum.
+        auto fold = [&]() {
             if constexpr (COMP) {
                 dd_add_d(t_hi, t_lo, blk);
                 t_abs += blk_abs;
             } else {
                 t_hi += blk;
             }
+            blk = 0.0;
+            blk_abs = 0.0;
+        };
+        if constexpr (U == 1) {
+            // Blocks of 2 steps: k = 2m flips column ctz(m)+1, k = 2m+1 flips
+            // column 0.  A uniform direction selects the +A or -A plane, so a
+            // flip is N plain v_add_f64 with SGPR operands; only block
+            // m = nblk/2 (column L-1) has a per-lane direction and uses FMA.
+            // Software pipeline: the next column's scalar loads are issued as
+            // soon as the current flip has consumed its SGPRs, and the step's
+            // N-1 multiplies cover their latency; the fences stop the
+            // scheduler from hoisting loads further (SGPR file overflow).
+            const double* Mneg = M + N * N;
+            auto a_col = [&](uint32_t m) {  // column of step k = 2m, uniform direction
+                const int tz = __builtin_ctz(m);
+                return (((m >> (tz + 1)) & 1u) ? Mneg : M) + (tz + 1) * N;
+            };
+            const uint32_t half = nblk >> 1;
+            constexpr uint32_t G = kFoldSteps / 2;
+            Column<N> ca, cb;
+            add_term<N, COMP>(x, false, blk, blk_abs);  // k = 0: the chunk start
+            load_column(cb, M);                          // k = 1: +column 0
+            add_column(x, cb);
+            load_column(ca, a_col(1));
+            step_fence();
+            add_term<N, COMP>(x, true, blk, blk_abs);
+            step_fence();
+            // One block; SPECIAL marks block `half`, whose column L-1 flips in a
+            // per-lane direction.  Kept branch-free: the loop is split there.
+            auto block = [&](uint32_t m, auto SPECIAL) {
+                if constexpr (decltype(SPECIAL)::value) fma_column(x, ca, lane_sign);
+                else add_column(x, ca);
+                load_column(cb, (m & 1u) ? Mneg : M);  // step 2m+1: column 0
+                step_fence();
+                add_term<N, COMP>(x, false, blk, blk_abs);
+                step_fence();
+                add_column(x, cb);
+                load_column(ca, a_col(m + 1));  // m+1 = nblk loads column L <= N-1: harmless
+                step_fence();
+                add_term<N, COMP>(x, true, blk, blk_abs);
+                step_fence();
+            };
+            auto run = [&](uint32_t m, uint32_t end) {
+                while (m < end) {
+                    const uint32_t mend = min(end, (m | (G - 1)) + 1);
+#pragma nounroll
+                    for (; m < mend; ++m) block(m, std::false_type{});
+                    fold();
+                }
+            };
+            run(1, half);
+            block(half, std::true_type{});
+            run(half + 1, nblk);
+            fold();
+        } else {
+            // Unpipelined walk (U = 0: tiny orders; U >= 2: tuning builds).
+            // Block m starts with the flip of the runtime column ctz(m) + U;
+            // its other steps flip compile-time columns.
+            constexpr uint32_t G = (kFoldSteps >> U) > 0 ? (kFoldSteps >> U) : 1;
+            auto block_tail = [&](uint32_t m) {
+                if constexpr (U > 0) {
+                    // bit U of the step index.  Held in a VGPR: a VOP3 op takes
+                    // one SGPR source, and that slot belongs to the column value.
+                    double s_mid = (m & 1u) ? -1.0 : 1.0;
+                    asm("" : "+v"(s_mid));
+                    static_for<1, (1 << U)>([&](auto J) {
+                        constexpr int j = decltype(J)::value;
+                        constexpr int c = __builtin_ctz(j);
+                        if constexpr (c + 1 < U) {
+                            flip_const<N, ((j >> (c + 1)) & 1) != 0>(x, M + c * N);
+                        } else {
+                            flip_signed<N>(x, M + c * N, s_mid);
+                        }
+                        add_term<N, COMP>(x, (j & 1) != 0, blk, blk_abs);
+                    });
+                }
+            };
+            add_term<N, COMP>(x, false, blk, blk_abs);  // block 0: the chunk start
+            block_tail(0u);
+            uint32_t m = 1;
+            while (m < nblk) {
+                const uint32_t mend = min(nblk, (m | (G - 1)) + 1);
+#pragma nounroll
+                for (; m < mend; ++m) {
+                    const int tz = __builtin_ctz(m);
+                    const int c = tz + U;
+                    const double s = (c + 1 < L) ? (((m >> (tz + 1)) & 1u) ? -1.0 : 1.0) : lane_sign;
+                    flip_signed<N>(x, M + c * N, s);
+                    add_term<N, COMP>(x, U == 0 && (m & 1u), blk, blk_abs);
+                    block_tail(m);
+                }
+                fold();
+            }
+            fold();
         }
         if (active) {
             if constexpr (COMP) {
@@ -238,8 +409,9 @@ reduce_partials_kernel(const double* __restrict__ partial, int nrec, int stride,
 template <int N, bool COMP>
 const void* dense_f64_ptr()
 {
-    constexpr int U = (N >= 5) ? 3 : 0;
-    return reinterpret_cast<const void*>(&dense_f64_kernel<N, U, COMP>);
+    constexpr int U = kDenseUnroll<N>;
+    if constexpr (SPM_DENSE_ONLY_N != 0 && N != SPM_DENSE_ONLY_N) return nullptr;
+    else return reinterpret_cast<const void*>(&dense_f64_kernel<N, U, COMP>);
 }
 
 template <int... Is>
@@ -256,7 +428,10 @@ const void* dense_f64_kernel_ptr(int n, bool compensated)
     return dense_f64_lookup(n, compensated, std::make_integer_sequence<int, kMaxN>{});
 }
 
-int dense_f64_unroll_bits(int n) { return n >= 5 ? 3 : 0; }
+int dense_f64_unroll_bits(int n)
+{
+    return (n >= SPM_DENSE_UNROLL + 2 && n <= SPM_DENSE_PREFETCH_MAX) ? SPM_DENSE_UNROLL : 0;
+}
 
 const void* reduce_partials_kernel_ptr() { return reinterpret_cast<const void*>(&reduce_partials_kernel); }
 

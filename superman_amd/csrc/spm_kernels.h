@@ -12,7 +12,7 @@ constexpr int kMaxN = 63;          // 64-bit Gray index (reference README.md:75)
 constexpr int kDenseBlock = 256;   // threads per block of the dense kernels
 
 // dense_f64.hip
-//   args: (const double* M_colmajor, const double* x0, double* partial,
+//   args: (const double* M /* [2][n][n]: +A, -A column-major */, const double* x0, double* partial,
 //          uint64_t task_begin, uint64_t task_end, uint64_t num_chunks, int L,
 //          int accumulate)
 const void* dense_f64_kernel_ptr(int n, bool compensated);

@@ -4,11 +4,12 @@ dense_f64_kernel), checked against the geometry the library actually picks.
 For every order n and every chunk size 2^L the planner can choose, the
 model replays the kernel's schedule -- chunk-start reconstruction from
 G(h 2^L), the runtime column ctz(m)+U at block starts (direction from bit
-ctz(m)+1 of m, or bit 0 of h for column L-1), the compile-time columns
-ctz(j) inside a block (direction from bit c+1 of j, or bit 0 of m for column
-U-1) and the term signs -- and asserts that each of the 2^(n-1) subsets is
-visited exactly once, every flip goes the right way and every term carries
-(-1)^|S|.  The kernel's own arithmetic is checked on the GPU."""
+ctz(m)+1 of m -- the +A / -A plane choice -- or bit 0 of h for column L-1),
+the compile-time columns ctz(j) inside a block (direction from bit c+1 of j,
+or bit 0 of m for column U-1) and the term signs -- and asserts that each of
+the 2^(n-1) subsets is visited exactly once, every flip goes the right way
+and every term carries (-1)^|S|.  The kernel's own arithmetic is checked on
+the GPU."""
 import ctypes
 
 import pytest
@@ -18,10 +19,6 @@ from superman_amd import _lib
 
 def ctz(x):
     return (x & -x).bit_length() - 1
-
-
-def unroll_bits(n):
-    return 3 if n >= 5 else 0
 
 
 def replay(n, L, U):
@@ -56,10 +53,15 @@ def replay(n, L, U):
 
 
 def geometry(n, waves):
-    L = ctypes.c_int()
+    L, U = ctypes.c_int(), ctypes.c_int()
     chunks, tasks = ctypes.c_uint64(), ctypes.c_uint64()
-    _lib.check(_lib.lib().spm_gray_geometry(n, waves, ctypes.byref(L), ctypes.byref(chunks), ctypes.byref(tasks)))
-    return L.value, chunks.value, tasks.value
+    _lib.check(_lib.lib().spm_gray_geometry(n, waves, ctypes.byref(L), ctypes.byref(chunks), ctypes.byref(tasks),
+                                            ctypes.byref(U)))
+    return L.value, chunks.value, tasks.value, U.value
+
+
+def unroll_bits(n):
+    return geometry(n, 1)[3]
 
 
 @pytest.mark.parametrize("n", range(1, 16))
@@ -75,8 +77,7 @@ def test_every_admissible_chunk_size(n):
 @pytest.mark.parametrize("waves", [1, 64, 3072, 4096, 1 << 20])
 def test_planner_geometry_is_admissible(waves):
     for n in range(1, 64):
-        L, chunks, tasks = geometry(n, waves)
-        U = unroll_bits(n)
+        L, chunks, tasks, U = geometry(n, waves)
         assert (L >= U + 1) if U else (L >= (1 if n >= 2 else 0))
         assert L <= n - 1 and L - U <= 30
         assert chunks == 1 << (n - 1 - L)
