@@ -80,10 +80,19 @@ int spm_device_info(int device, int* compute_units, int* clock_khz, char* name, 
  * synchronous with respect to it. */
 int spm_dense_f64(const double* a, int n, int precision, int device, void* stream, spm_result* out);
 
+/* ---- dense complex128 ----------------------------------------------------
+ * Replaces gpuComputeComplex (Algorithms/Complex/gpuComputeComplex.cu:199-418,
+ * kernels :39-197), reached through complexPerman (Algorithms/AllAlgorithms.h:
+ * 93-110).  `a` is row-major interleaved (re, im), i.e. std::complex<double>
+ * storage; the result is in re/im (the reference prints it and returns
+ * void).  1 <= n <= 63. */
+int spm_dense_c128(const double* a, int n, int precision, int device, void* stream, spm_result* out);
+
 /* Device-resident plans: the matrix is uploaded once, then spm_plan_run
  * computes with inputs already in HBM (what the benchmark times). */
 typedef struct spm_plan spm_plan;
 int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_plan** plan);
+int spm_plan_dense_c128(const double* a, int n, int precision, int device, spm_plan** plan);
 int spm_plan_run(spm_plan* plan, void* stream, spm_result* out);
 int spm_plan_destroy(spm_plan* plan);
 /* Launch geometry of a plan: grid blocks, block threads, L (Gray steps per

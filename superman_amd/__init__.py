@@ -5,7 +5,8 @@ C ABI, include/superman_amd.h); this package is a thin Python host layer over
 it.  Importing the package does not load the library; the first call does,
 and raises if the library has not been built (`python -m superman_amd.build`).
 """
-from .api import (DensePlan, Permanent, dense_f64, device_count, device_info, permanent,  # noqa: F401
-                  synchronize)
+from .api import (DensePlan, Permanent, dense_c128, dense_f64, device_count, device_info,  # noqa: F401
+                  permanent, synchronize)
 
-__all__ = ["DensePlan", "Permanent", "dense_f64", "device_count", "permanent"]
+__all__ = ["DensePlan", "Permanent", "dense_c128", "dense_f64", "device_count", "device_info", "permanent",
+           "synchronize"]

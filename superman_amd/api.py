@@ -79,28 +79,46 @@ def dense_f64(a, precision="kahan", device: int = 0, stream: int | None = None) 
     return _to_perm(r, False)
 
 
+def dense_c128(a, precision="kahan", device: int = 0, stream: int | None = None) -> Permanent:
+    a = _square(a, np.complex128)
+    r = SpmResult()
+    st = lib().spm_dense_c128(a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0],
+                              _prec(precision), device, stream, ctypes.byref(r))
+    check(st, "spm_dense_c128")
+    return _to_perm(r, True)
+
+
 def permanent(a, precision="kahan", device: int = 0) -> Permanent:
-    """Permanent of a square matrix on the GPU."""
+    """Permanent of a square matrix on the GPU (complex -> complex128 path,
+    otherwise dense FP64)."""
     a = np.asarray(a)
     if np.iscomplexobj(a):
-        raise NotImplementedError("complex path")
+        return dense_c128(a, precision, device)
     return dense_f64(a, precision, device)
 
 
 class DensePlan:
-    """A dense FP64 permanent with the matrix resident on the device."""
+    """A dense permanent (FP64, or complex128 for a complex matrix) with the
+    matrix resident on the device."""
 
     def __init__(self, a, precision="kahan", device: int = 0):
-        a = _square(a, np.float64)
+        self.complex = np.iscomplexobj(a)
         self._h = ctypes.c_void_p()
-        check(lib().spm_plan_dense_f64(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0],
-                                       _prec(precision), device, ctypes.byref(self._h)), "spm_plan_dense_f64")
+        if self.complex:
+            a = _square(a, np.complex128)
+            check(lib().spm_plan_dense_c128(a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            a.shape[0], _prec(precision), device, ctypes.byref(self._h)),
+                  "spm_plan_dense_c128")
+        else:
+            a = _square(a, np.float64)
+            check(lib().spm_plan_dense_f64(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0],
+                                           _prec(precision), device, ctypes.byref(self._h)), "spm_plan_dense_f64")
         self.n = a.shape[0]
 
     def run(self, stream: int | None = None) -> Permanent:
         r = SpmResult()
         check(lib().spm_plan_run(self._h, stream, ctypes.byref(r)), "spm_plan_run")
-        return _to_perm(r, False)
+        return _to_perm(r, self.complex)
 
     def info(self) -> dict:
         g, b, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

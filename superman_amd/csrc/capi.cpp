@@ -126,7 +126,7 @@ using namespace spm;
 // Plans
 // ---------------------------------------------------------------------------
 struct spm_plan {
-    int kind = 0;  // 1 = dense f64
+    int kind = 0;  // 1 = dense f64, 2 = dense c128
     int n = 0;
     int precision = 0;
     int device = 0;
@@ -134,6 +134,8 @@ struct spm_plan {
     int L = 0;
     uint64_t chunks = 0, tasks = 0;
     int slices = 1;
+    int rec = 4;    // doubles per per-block partial record
+    int comps = 1;  // double-double components per record (1 real, 2 complex)
     const void* kern = nullptr;
     double* d_M = nullptr;
     double* d_x0 = nullptr;
@@ -166,13 +168,17 @@ static void plan_free(spm_plan* p)
         }                                              \
     } while (0)
 
-extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_plan** out)
+// Dense plans, real (kind 1) and complex (kind 2).  `a` is row-major n x n,
+// doubles or interleaved (re, im) pairs.
+static int make_dense_plan(int kind, const double* a, int n, int precision, int device, spm_plan** out)
 {
     if (!out) return SPM_EINVAL;
     *out = nullptr;
-    if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
+    const int maxn = kind == 1 ? kMaxN : kMaxComplexN;
+    const int w = kind == 1 ? 1 : 2;  // doubles per entry
+    if (!a || n < 1 || n > maxn) return SPM_EINVAL;
     if (precision != SPM_PREC_PLAIN && precision != SPM_PREC_COMPENSATED) return SPM_EINVAL;
-    for (int e = 0; e < n * n; ++e)
+    for (int e = 0; e < w * n * n; ++e)
         if (!std::isfinite(a[e])) return SPM_EINVAL;
     int cus = 0;
     int st = device_cus(device, &cus);
@@ -181,11 +187,14 @@ extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int dev
     if (!guard.ok()) return SPM_EHIP;
 
     spm_plan* p = new spm_plan;
-    p->kind = 1;
+    p->kind = kind;
     p->n = n;
     p->precision = precision;
     p->device = device;
-    p->kern = dense_f64_kernel_ptr(n, precision == SPM_PREC_COMPENSATED);
+    const bool comp = precision == SPM_PREC_COMPENSATED;
+    p->kern = kind == 1 ? dense_f64_kernel_ptr(n, comp) : dense_c128_kernel_ptr(n, comp);
+    p->rec = kind == 1 ? 4 : 8;
+    p->comps = kind == 1 ? 1 : 2;
     if (!p->kern) {
         plan_free(p);
         return SPM_EINVAL;
@@ -197,42 +206,62 @@ extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int dev
         return st;
     }
     const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
-    gray_geometry(n, dense_f64_unroll_bits(n), waves, &p->L, &p->chunks, &p->tasks);
+    const int ub = kind == 1 ? dense_f64_unroll_bits(n) : (n >= 3 ? 1 : 0);
+    gray_geometry(n, ub, waves, &p->L, &p->chunks, &p->tasks);
     const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
     p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
     // Split very long runs (~> 4 s of VALU work at 64 lanes x 256 CUs x 2.4 GHz)
     // into several launches that accumulate into the same per-block records.
-    const double est_s = std::ldexp(1.0, n - 1) * (2.0 * n) / 3.9e13;
+    const double ops = kind == 1 ? 2.0 * n : 6.0 * n;
+    const double est_s = std::ldexp(1.0, n - 1) * ops / 3.9e13;
     p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
     char kn[96];
-    std::snprintf(kn, sizeof kn, "dense_f64_kernel<%d, %d, %s>", n, dense_f64_unroll_bits(n),
-                  precision == SPM_PREC_COMPENSATED ? "true" : "false");
+    if (kind == 1)
+        std::snprintf(kn, sizeof kn, "dense_f64_kernel<%d, %d, %s>", n, ub, comp ? "true" : "false");
+    else
+        std::snprintf(kn, sizeof kn, "dense_c128_kernel<%d, %s>", n, comp ? "true" : "false");
     p->kname = kn;
 
-    // Host preparation (dpSingleGPU.cuh:62-82), plus a negated copy of the
-    // column-major matrix: the kernel picks the plane by flip direction.
-    const size_t nn = (size_t)n * n;
-    std::vector<double> colmaj(2 * nn), x0(n);
+    // Host preparation (dpSingleGPU.cuh:62-82; gpuComputeComplex.cu:212-241):
+    // x(empty) = a[i][n-1] - rowsum_i / 2 with the row sum taken left to right
+    // (re and im separately), and the column-major matrix with a negated copy
+    // (the kernel picks the plane by flip direction).
+    const size_t col = (size_t)w * n;   // doubles per column
+    const size_t plane = col * n;
+    std::vector<double> colmaj(2 * plane), x0(col);
     for (int i = 0; i < n; ++i) {
-        double rowSum = 0;
-        for (int j = 0; j < n; ++j) rowSum += a[i * n + j];
-        x0[i] = a[i * n + (n - 1)] - (rowSum / 2);
-        for (int j = 0; j < n; ++j) {
-            colmaj[(size_t)j * n + i] = a[i * n + j];
-            colmaj[nn + (size_t)j * n + i] = -a[i * n + j];
+        for (int c = 0; c < w; ++c) {
+            double rowSum = 0;
+            for (int j = 0; j < n; ++j) rowSum += a[(size_t)w * (i * n + j) + c];
+            x0[(size_t)w * i + c] = a[(size_t)w * (i * n + n - 1) + c] - (rowSum / 2);
+            for (int j = 0; j < n; ++j) {
+                const double v = a[(size_t)w * (i * n + j) + c];
+                colmaj[j * col + (size_t)w * i + c] = v;
+                colmaj[plane + j * col + (size_t)w * i + c] = -v;
+            }
         }
     }
-    SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * nn));
-    SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * n));
-    SPM_TRY(hipMalloc(&p->d_partial, sizeof(double) * 4 * p->grid));
-    SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 4));
-    SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 4, hipHostMallocDefault));
-    SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice));
-    SPM_TRY(hipMemcpy(p->d_x0, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * plane));
+    SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * col));
+    SPM_TRY(hipMalloc(&p->d_partial, sizeof(double) * p->rec * p->grid));
+    SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 6));
+    SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 6, hipHostMallocDefault));
+    SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(double) * 2 * plane, hipMemcpyHostToDevice));
+    SPM_TRY(hipMemcpy(p->d_x0, x0.data(), sizeof(double) * col, hipMemcpyHostToDevice));
     SPM_TRY(hipEventCreate(&p->ev0));
     SPM_TRY(hipEventCreate(&p->ev1));
     *out = p;
     return SPM_OK;
+}
+
+extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_plan** out)
+{
+    return make_dense_plan(1, a, n, precision, device, out);
+}
+
+extern "C" int spm_plan_dense_c128(const double* a, int n, int precision, int device, spm_plan** out)
+{
+    return make_dense_plan(2, a, n, precision, device, out);
 }
 
 #undef SPM_TRY
@@ -254,13 +283,13 @@ static int dense_run(spm_plan* p, hipStream_t st, spm_result* r)
         ++launches;
     }
     {
-        int nrec = p->grid, stride = 4, comps = 1;
+        int nrec = p->grid, stride = p->rec, comps = p->comps;
         void* args[] = {&p->d_partial, &nrec, &stride, &comps, &p->d_out};
         if (hipLaunchKernel(reduce_partials_kernel_ptr(), dim3(1), dim3(256), args, 0, st) != hipSuccess)
             return SPM_EHIP;
     }
     if (hipEventRecord(p->ev1, st) != hipSuccess) return SPM_EHIP;
-    if (hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * 3, hipMemcpyDeviceToHost, st) != hipSuccess)
+    if (hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * 3 * p->comps, hipMemcpyDeviceToHost, st) != hipSuccess)
         return SPM_EHIP;
     if (hipStreamSynchronize(st) != hipSuccess) return SPM_EHIP;
     float ms = 0.f;
@@ -273,9 +302,14 @@ static int dense_run(spm_plan* p, hipStream_t st, spm_result* r)
     r->hi_re = f * hi;
     r->lo_re = f * lo;
     r->re = f * (hi + lo);
-    r->im = 0.0;
     r->abs_sum = 2.0 * p->h_out[2];
-    r->err_est = p->precision == SPM_PREC_COMPENSATED ? dense_err_est(n, r->abs_sum) : -1.0;
+    if (p->comps == 2) {
+        r->hi_im = f * p->h_out[3];
+        r->lo_im = f * p->h_out[4];
+        r->im = f * (p->h_out[3] + p->h_out[4]);
+    }
+    if (p->precision != SPM_PREC_COMPENSATED) r->err_est = -1.0;
+    else r->err_est = p->kind == 1 ? dense_err_est(n, r->abs_sum) : complex_err_est(n, r->abs_sum);
     r->status = SPM_OK;
     r->kernel_ms = ms;
     r->subsets = uint64_t(1) << (n - 1);
@@ -288,7 +322,7 @@ static int dense_run(spm_plan* p, hipStream_t st, spm_result* r)
 extern "C" int spm_plan_run(spm_plan* p, void* stream, spm_result* out)
 {
     if (!p || !out) return SPM_EINVAL;
-    if (p->kind == 1) return dense_run(p, (hipStream_t)stream, out);
+    if (p->kind == 1 || p->kind == 2) return dense_run(p, (hipStream_t)stream, out);
     return SPM_EINVAL;
 }
 
@@ -310,12 +344,13 @@ extern "C" int spm_plan_info(const spm_plan* p, int* grid, int* block, int* log2
 
 extern "C" const char* spm_plan_kernel_name(const spm_plan* p) { return p ? p->kname.c_str() : ""; }
 
-extern "C" int spm_dense_f64(const double* a, int n, int precision, int device, void* stream, spm_result* out)
+static int dense_oneshot(int kind, const double* a, int n, int precision, int device, void* stream,
+                         spm_result* out)
 {
     if (!out) return SPM_EINVAL;
     const double t0 = now_seconds();
     spm_plan* p = nullptr;
-    int st = spm_plan_dense_f64(a, n, precision, device, &p);
+    int st = make_dense_plan(kind, a, n, precision, device, &p);
     if (st != SPM_OK) {
         std::memset(out, 0, sizeof *out);
         out->status = st;
@@ -326,6 +361,16 @@ extern "C" int spm_dense_f64(const double* a, int n, int precision, int device, 
     if (st == SPM_OK) out->seconds = now_seconds() - t0;
     else out->status = st;
     return st;
+}
+
+extern "C" int spm_dense_f64(const double* a, int n, int precision, int device, void* stream, spm_result* out)
+{
+    return dense_oneshot(1, a, n, precision, device, stream, out);
+}
+
+extern "C" int spm_dense_c128(const double* a, int n, int precision, int device, void* stream, spm_result* out)
+{
+    return dense_oneshot(2, a, n, precision, device, stream, out);
 }
 
 // ---------------------------------------------------------------------------

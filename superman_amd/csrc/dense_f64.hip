@@ -252,7 +252,7 @@ dense_f64_kernel(const double* __restrict__ M,   // [2][N][N]: +A then -A, colum
         double blk = 0.0, blk_abs = 0.0;
         // Block sums accumulate in plain FP64 for kFoldSteps steps, then fold
         // into the compensated per-lane sum.
-        auto fold = [&]() {
+        auto fold = [&]() __attribute__((always_inline)) {
             if constexpr (COMP) {
                 dd_add_d(t_hi, t_lo, blk);
                 t_abs += blk_abs;
@@ -272,7 +272,7 @@ dense_f64_kernel(const double* __restrict__ M,   // [2][N][N]: +A then -A, colum
             // N-1 multiplies cover their latency; the fences stop the
             // scheduler from hoisting loads further (SGPR file overflow).
             const double* Mneg = M + N * N;
-            auto a_col = [&](uint32_t m) {  // column of step k = 2m, uniform direction
+            auto a_col = [&](uint32_t m) __attribute__((always_inline)) {  // column of step k = 2m, uniform direction
                 const int tz = __builtin_ctz(m);
                 return (((m >> (tz + 1)) & 1u) ? Mneg : M) + (tz + 1) * N;
             };
@@ -288,7 +288,7 @@ dense_f64_kernel(const double* __restrict__ M,   // [2][N][N]: +A then -A, colum
             step_fence();
             // One block; SPECIAL marks block `half`, whose column L-1 flips in a
             // per-lane direction.  Kept branch-free: the loop is split there.
-            auto block = [&](uint32_t m, auto SPECIAL) {
+            auto block = [&](uint32_t m, auto SPECIAL) __attribute__((always_inline)) {
                 if constexpr (decltype(SPECIAL)::value) fma_column(x, ca, lane_sign);
                 else add_column(x, ca);
                 load_column(cb, (m & 1u) ? Mneg : M);  // step 2m+1: column 0
@@ -301,7 +301,7 @@ dense_f64_kernel(const double* __restrict__ M,   // [2][N][N]: +A then -A, colum
                 add_term<N, COMP>(x, true, blk, blk_abs);
                 step_fence();
             };
-            auto run = [&](uint32_t m, uint32_t end) {
+            auto run = [&](uint32_t m, uint32_t end) __attribute__((always_inline)) {
                 while (m < end) {
                     const uint32_t mend = min(end, (m | (G - 1)) + 1);
 #pragma nounroll
@@ -318,13 +318,13 @@ dense_f64_kernel(const double* __restrict__ M,   // [2][N][N]: +A then -A, colum
             // Block m starts with the flip of the runtime column ctz(m) + U;
             // its other steps flip compile-time columns.
             constexpr uint32_t G = (kFoldSteps >> U) > 0 ? (kFoldSteps >> U) : 1;
-            auto block_tail = [&](uint32_t m) {
+            auto block_tail = [&](uint32_t m) __attribute__((always_inline)) {
                 if constexpr (U > 0) {
                     // bit U of the step index.  Held in a VGPR: a VOP3 op takes
                     // one SGPR source, and that slot belongs to the column value.
                     double s_mid = (m & 1u) ? -1.0 : 1.0;
                     asm("" : "+v"(s_mid));
-                    static_for<1, (1 << U)>([&](auto J) {
+                    static_for<1, (1 << U)>([&](auto J) __attribute__((always_inline)) {
                         constexpr int j = decltype(J)::value;
                         constexpr int c = __builtin_ctz(j);
                         if constexpr (c + 1 < U) {

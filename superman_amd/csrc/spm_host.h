@@ -24,6 +24,15 @@ inline double dense_err_est(int n, double abs_sum)
     return 2.0 * n * u * abs_sum;
 }
 
+// The same for complex runs: abs_sum = 2 sum |p|_1 |x_last|_1 bounds the
+// terms' moduli; each term carries ~2(n-1) roundings from its complex
+// products plus its factors' incremental updates.
+inline double complex_err_est(int n, double abs_sum)
+{
+    const double u = 1.1102230246251565e-16;
+    return 4.0 * n * u * abs_sum;
+}
+
 class DeviceGuard {
 public:
     explicit DeviceGuard(int device);

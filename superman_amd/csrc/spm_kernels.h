@@ -20,4 +20,10 @@ int dense_f64_unroll_bits(int n);
 //   args: (const double* partial, int nrec, int stride, int comps, double* out)
 const void* reduce_partials_kernel_ptr();
 
+// dense_c128.hip -- same arguments as the real kernel; M is [2][n][2n]
+// (+A, -A; column-major, interleaved re/im), x0 is 2n doubles, and each
+// partial record is 8 doubles (re hi, lo, abs, im hi, lo, 0, 0, 0).
+constexpr int kMaxComplexN = 63;
+const void* dense_c128_kernel_ptr(int n, bool compensated);
+
 }  // namespace spm
