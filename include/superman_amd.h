@@ -88,11 +88,27 @@ int spm_dense_f64(const double* a, int n, int precision, int device, void* strea
  * void).  1 <= n <= 63. */
 int spm_dense_c128(const double* a, int n, int precision, int device, void* stream, spm_result* out);
 
+/* ---- exact integer ---------------------------------------------------------
+ * Integer matrices (|a_ij| <= 2^30, row sums of |a_ij| < 2^30) computed
+ * exactly: y = 2x in int32, T = sum (-1)^k prod y (mod 2^128), perm =
+ * (-1)^(n-1) T / 2^(n-1), returned in exact_lo/exact_hi (is_exact = 1) and
+ * rounded into `re`.  Returns SPM_ERANGE unless 2^(n-1) times a bound on
+ * |perm| (Bregman-Minc for 0/1 matrices, the product of absolute row sums
+ * otherwise) stays below 2^126.
+ * The reference has no integer path: its sparse drivers gpuSPSingleGPU
+ * (Algorithms/GPU/GPUKernelWrappers.cu:65-113) -> spSingleGPU
+ * (Algorithms/GPU/Sparse/spSingleGPU.cuh:44-234) and the generated-kernel
+ * driver KernelGenSingleGPU (Algorithms/GPU/CodeGeneration/
+ * KernelGenSingleGPU.cuh:47-197) compute 0/1 and integer inputs in FP64;
+ * this entry point replaces them for integer-valued matrices. */
+int spm_int_exact(const int64_t* a, int n, int device, void* stream, spm_result* out);
+
 /* Device-resident plans: the matrix is uploaded once, then spm_plan_run
  * computes with inputs already in HBM (what the benchmark times). */
 typedef struct spm_plan spm_plan;
 int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_plan** plan);
 int spm_plan_dense_c128(const double* a, int n, int precision, int device, spm_plan** plan);
+int spm_plan_int(const int64_t* a, int n, int device, spm_plan** plan);
 int spm_plan_run(spm_plan* plan, void* stream, spm_result* out);
 int spm_plan_destroy(spm_plan* plan);
 /* Launch geometry of a plan: grid blocks, block threads, L (Gray steps per

@@ -6,7 +6,7 @@ it.  Importing the package does not load the library; the first call does,
 and raises if the library has not been built (`python -m superman_amd.build`).
 """
 from .api import (DensePlan, Permanent, dense_c128, dense_f64, device_count, device_info,  # noqa: F401
-                  permanent, synchronize)
+                  int_exact, is_integral, permanent, synchronize)
 
-__all__ = ["DensePlan", "Permanent", "dense_c128", "dense_f64", "device_count", "device_info", "permanent",
-           "synchronize"]
+__all__ = ["DensePlan", "Permanent", "dense_c128", "dense_f64", "device_count", "device_info", "int_exact",
+           "is_integral", "permanent", "synchronize"]

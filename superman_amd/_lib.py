@@ -69,6 +69,8 @@ PROTOTYPES = {
                                        ctypes.c_char_p, ctypes.c_int]),
     "spm_dense_f64": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_dense_c128": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
+    "spm_int_exact": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, _VP, _R]),
+    "spm_plan_int": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
     "spm_plan_dense_f64": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
     "spm_plan_dense_c128": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
     "spm_plan_run": (ctypes.c_int, [_VP, _VP, _R]),

@@ -88,23 +88,63 @@ def dense_c128(a, precision="kahan", device: int = 0, stream: int | None = None)
     return _to_perm(r, True)
 
 
-def permanent(a, precision="kahan", device: int = 0) -> Permanent:
-    """Permanent of a square matrix on the GPU (complex -> complex128 path,
-    otherwise dense FP64)."""
+def int_exact(a, device: int = 0, stream: int | None = None) -> Permanent:
+    """Exact permanent of an integer matrix (result in .exact)."""
+    a = np.asarray(a)
+    if a.dtype.kind == "f":
+        if not np.all(np.isfinite(a)) or not np.all(a == np.round(a)):
+            raise ValueError("int_exact needs integer entries")
+    a = _square(a, np.int64)
+    r = SpmResult()
+    st = lib().spm_int_exact(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.shape[0], device, stream,
+                             ctypes.byref(r))
+    check(st, "spm_int_exact")
+    return _to_perm(r, False)
+
+
+def is_integral(a) -> bool:
     a = np.asarray(a)
     if np.iscomplexobj(a):
-        return dense_c128(a, precision, device)
+        return False
+    if a.dtype.kind in "iub":
+        return True
+    return bool(np.all(np.isfinite(a)) and np.all(a == np.round(a)) and np.all(np.abs(a) <= 2 ** 30))
+
+
+def permanent(a, precision="auto", device: int = 0) -> Permanent:
+    """Permanent of a square matrix on the GPU.
+
+    precision: "auto" (integer matrices exactly when their bound allows, else
+    compensated FP64 / complex128), "exact", "kahan" (compensated) or "dd"
+    (plain FP64, the reference's name)."""
+    a = np.asarray(a)
+    if np.iscomplexobj(a):
+        return dense_c128(a, "kahan" if precision in ("auto", "exact") else precision, device)
+    if precision == "exact":
+        return int_exact(a, device)
+    if precision == "auto":
+        if is_integral(a):
+            try:
+                return int_exact(a, device)
+            except _lib.SpmError as e:
+                if e.status != _lib.SPM_ERANGE:
+                    raise
+        precision = "kahan"
     return dense_f64(a, precision, device)
 
 
 class DensePlan:
-    """A dense permanent (FP64, or complex128 for a complex matrix) with the
-    matrix resident on the device."""
+    """A permanent with the matrix resident on the device: FP64, complex128
+    for a complex matrix, or exact integer with precision="exact"."""
 
     def __init__(self, a, precision="kahan", device: int = 0):
         self.complex = np.iscomplexobj(a)
         self._h = ctypes.c_void_p()
-        if self.complex:
+        if precision == "exact":
+            a = _square(a, np.int64)
+            check(lib().spm_plan_int(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.shape[0], device,
+                                     ctypes.byref(self._h)), "spm_plan_int")
+        elif self.complex:
             a = _square(a, np.complex128)
             check(lib().spm_plan_dense_c128(a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                             a.shape[0], _prec(precision), device, ctypes.byref(self._h)),

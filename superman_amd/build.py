@@ -35,6 +35,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     ("dense_f64.hip", []),
     ("dense_c128.hip", []),
+    ("int_dense.hip", []),
     ("capi.cpp", []),
 ]
 LINK_LIBS = ["-lhiprtc", "-lpthread"]

@@ -126,7 +126,7 @@ using namespace spm;
 // Plans
 // ---------------------------------------------------------------------------
 struct spm_plan {
-    int kind = 0;  // 1 = dense f64, 2 = dense c128
+    int kind = 0;  // 1 = dense f64, 2 = dense c128, 3 = exact integer
     int n = 0;
     int precision = 0;
     int device = 0;
@@ -137,8 +137,9 @@ struct spm_plan {
     int rec = 4;    // doubles per per-block partial record
     int comps = 1;  // double-double components per record (1 real, 2 complex)
     const void* kern = nullptr;
-    double* d_M = nullptr;
+    double* d_M = nullptr;   // dense kinds; the integer kind stores int32 here
     double* d_x0 = nullptr;
+    double log2_bound = 0;   // integer kind: log2 of the bound on |T|
     double* d_partial = nullptr;
     double* d_out = nullptr;
     double* h_out = nullptr;
@@ -254,9 +255,95 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     return SPM_OK;
 }
 
+// Exact integer plans (kind 3): y = 2x in int32, T mod 2^128.
+static int make_int_plan(const int64_t* a, int n, int device, spm_plan** out)
+{
+    if (!out) return SPM_EINVAL;
+    *out = nullptr;
+    if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
+    // |y_i| <= 2 sum_j |a_ij| must fit int32.
+    std::vector<int> y0(n);
+    bool binary = true;
+    double lb = n - 1;  // log2 bound on |T| = 2^(n-1) |perm|
+    for (int i = 0; i < n; ++i) {
+        int64_t rs = 0, ra = 0;
+        for (int j = 0; j < n; ++j) {
+            const int64_t v = a[i * n + j];
+            if (v > (int64_t(1) << 30) || v < -(int64_t(1) << 30)) return SPM_ERANGE;
+            rs += v;
+            ra += v < 0 ? -v : v;
+            binary = binary && (v == 0 || v == 1);
+        }
+        if (ra >= (int64_t(1) << 30)) return SPM_ERANGE;
+        y0[i] = (int)(2 * a[i * n + n - 1] - rs);
+        if (ra == 0) lb = -1e300;  // a zero row: perm = 0
+        else lb += binary ? std::lgamma((double)ra + 1.0) / ((double)ra * std::log(2.0)) : std::log2((double)ra);
+    }
+    // Bregman-Minc (0/1) or row-sum-product bound; the mod-2^128 sum is T
+    // itself only when |T| < 2^127 (with margin for the log arithmetic).
+    if (lb >= 126.0) return SPM_ERANGE;
+    int cus = 0;
+    int st = device_cus(device, &cus);
+    if (st != SPM_OK) return st;
+    DeviceGuard guard(device);
+    if (!guard.ok()) return SPM_EHIP;
+
+    spm_plan* p = new spm_plan;
+    p->kind = 3;
+    p->n = n;
+    p->precision = -1;
+    p->device = device;
+    p->log2_bound = lb;
+    p->kern = int_dense_kernel_ptr(n);
+    p->rec = 2;
+    p->comps = 0;
+    if (!p->kern) {
+        plan_free(p);
+        return SPM_EINVAL;
+    }
+    int bpc = 0;
+    st = blocks_per_cu(device, p->kern, kDenseBlock, &bpc);
+    if (st != SPM_OK) {
+        plan_free(p);
+        return st;
+    }
+    const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
+    gray_geometry(n, n >= 3 ? 1 : 0, waves, &p->L, &p->chunks, &p->tasks);
+    const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
+    p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
+    const double est_s = std::ldexp(1.0, n - 1) * (2.0 * n) / 3.9e13;
+    p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
+    char kn[96];
+    std::snprintf(kn, sizeof kn, "int_dense_kernel<%d>", n);
+    p->kname = kn;
+    const size_t nn = (size_t)n * n;
+    std::vector<int> colmaj(2 * nn);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            colmaj[(size_t)j * n + i] = (int)(2 * a[i * n + j]);
+            colmaj[nn + (size_t)j * n + i] = (int)(-2 * a[i * n + j]);
+        }
+    SPM_TRY(hipMalloc(&p->d_M, sizeof(int) * 2 * nn));
+    SPM_TRY(hipMalloc(&p->d_x0, sizeof(int) * n));
+    SPM_TRY(hipMalloc(&p->d_partial, sizeof(unsigned long long) * 2 * p->grid));
+    SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 6));
+    SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 6, hipHostMallocDefault));
+    SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(int) * 2 * nn, hipMemcpyHostToDevice));
+    SPM_TRY(hipMemcpy(p->d_x0, y0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    SPM_TRY(hipEventCreate(&p->ev0));
+    SPM_TRY(hipEventCreate(&p->ev1));
+    *out = p;
+    return SPM_OK;
+}
+
 extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_plan** out)
 {
     return make_dense_plan(1, a, n, precision, device, out);
+}
+
+extern "C" int spm_plan_int(const int64_t* a, int n, int device, spm_plan** out)
+{
+    return make_int_plan(a, n, device, out);
 }
 
 extern "C" int spm_plan_dense_c128(const double* a, int n, int precision, int device, spm_plan** out)
@@ -319,10 +406,73 @@ static int dense_run(spm_plan* p, hipStream_t st, spm_result* r)
     return SPM_OK;
 }
 
+// Fill `r` from T mod 2^128 of an integer run.  T must be a multiple of
+// 2^(n-1) (it is 2^(n-1) (-1)^(n-1) perm); a violation means a corrupted
+// run and is reported as a HIP error.
+static int int_result(const spm_plan* p, unsigned long long lo, unsigned long long hi, spm_result* r)
+{
+    const int n = p->n;
+    const unsigned __int128 T = ((unsigned __int128)hi << 64) | lo;
+    const unsigned __int128 mask = n >= 2 ? (((unsigned __int128)1 << (n - 1)) - 1) : 0;
+    if (T & mask) return SPM_EHIP;
+    __int128 perm = ((__int128)T) >> (n - 1);
+    if ((n - 1) & 1) perm = -perm;
+    r->exact_lo = (int64_t)(uint64_t)(unsigned __int128)perm;
+    r->exact_hi = (int64_t)(uint64_t)(((unsigned __int128)perm) >> 64);
+    r->is_exact = 1;
+    r->re = (double)perm;
+    r->hi_re = r->re;
+    r->lo_re = (double)(perm - (__int128)r->re);
+    r->err_est = 0.0;
+    return SPM_OK;
+}
+
+static int int_run(spm_plan* p, hipStream_t st, spm_result* r)
+{
+    const double t0 = now_seconds();
+    DeviceGuard guard(p->device);
+    if (!guard.ok()) return SPM_EHIP;
+    if (hipEventRecord(p->ev0, st) != hipSuccess) return SPM_EHIP;
+    const uint64_t per = (p->tasks + p->slices - 1) / p->slices;
+    int launches = 0;
+    for (int s = 0; s < p->slices; ++s) {
+        uint64_t tb = per * s, te = std::min<uint64_t>(p->tasks, per * (s + 1));
+        if (tb >= te) break;
+        int acc = s > 0 ? 1 : 0;
+        void* args[] = {&p->d_M, &p->d_x0, &p->d_partial, &tb, &te, &p->chunks, &p->L, &acc};
+        if (hipLaunchKernel(p->kern, dim3(p->grid), dim3(kDenseBlock), args, 0, st) != hipSuccess) return SPM_EHIP;
+        ++launches;
+    }
+    {
+        int nrec = p->grid;
+        void* args[] = {&p->d_partial, &nrec, &p->d_out};
+        if (hipLaunchKernel(reduce_i128_kernel_ptr(), dim3(1), dim3(256), args, 0, st) != hipSuccess)
+            return SPM_EHIP;
+    }
+    if (hipEventRecord(p->ev1, st) != hipSuccess) return SPM_EHIP;
+    if (hipMemcpyAsync(p->h_out, p->d_out, 16, hipMemcpyDeviceToHost, st) != hipSuccess) return SPM_EHIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return SPM_EHIP;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p->ev0, p->ev1) != hipSuccess) ms = -1.f;
+    std::memset(r, 0, sizeof *r);
+    unsigned long long w[2];
+    std::memcpy(w, p->h_out, 16);
+    const int rc = int_result(p, w[0], w[1], r);
+    if (rc != SPM_OK) return rc;
+    r->status = SPM_OK;
+    r->kernel_ms = ms;
+    r->subsets = uint64_t(1) << (p->n - 1);
+    r->n = p->n;
+    r->launches = launches;
+    r->seconds = now_seconds() - t0;
+    return SPM_OK;
+}
+
 extern "C" int spm_plan_run(spm_plan* p, void* stream, spm_result* out)
 {
     if (!p || !out) return SPM_EINVAL;
     if (p->kind == 1 || p->kind == 2) return dense_run(p, (hipStream_t)stream, out);
+    if (p->kind == 3) return int_run(p, (hipStream_t)stream, out);
     return SPM_EINVAL;
 }
 
@@ -371,6 +521,24 @@ extern "C" int spm_dense_f64(const double* a, int n, int precision, int device, 
 extern "C" int spm_dense_c128(const double* a, int n, int precision, int device, void* stream, spm_result* out)
 {
     return dense_oneshot(2, a, n, precision, device, stream, out);
+}
+
+extern "C" int spm_int_exact(const int64_t* a, int n, int device, void* stream, spm_result* out)
+{
+    if (!out) return SPM_EINVAL;
+    const double t0 = now_seconds();
+    spm_plan* p = nullptr;
+    int st = make_int_plan(a, n, device, &p);
+    if (st != SPM_OK) {
+        std::memset(out, 0, sizeof *out);
+        out->status = st;
+        return st;
+    }
+    st = int_run(p, (hipStream_t)stream, out);
+    plan_free(p);
+    if (st == SPM_OK) out->seconds = now_seconds() - t0;
+    else out->status = st;
+    return st;
 }
 
 // ---------------------------------------------------------------------------

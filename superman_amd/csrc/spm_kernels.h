@@ -26,4 +26,12 @@ const void* reduce_partials_kernel_ptr();
 constexpr int kMaxComplexN = 63;
 const void* dense_c128_kernel_ptr(int n, bool compensated);
 
+// int_dense.hip -- exact integer walk.
+//   args: (const int* M /* [2][n][n]: +2A, -2A column-major */, const int* y0,
+//          unsigned long long* partial /* [grid][2] */, uint64_t task_begin,
+//          uint64_t task_end, uint64_t num_chunks, int L, int accumulate)
+const void* int_dense_kernel_ptr(int n);
+//   args: (const unsigned long long* partial, int nrec, unsigned long long* out /* [2] */)
+const void* reduce_i128_kernel_ptr();
+
 }  // namespace spm
