@@ -18,6 +18,7 @@
 #ifndef SUPERMAN_AMD_H
 #define SUPERMAN_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -103,12 +104,40 @@ int spm_dense_c128(const double* a, int n, int precision, int device, void* stre
  * this entry point replaces them for integer-valued matrices. */
 int spm_int_exact(const int64_t* a, int n, int device, void* stream, spm_result* out);
 
+/* ---- SpaRyser with per-pattern generated kernels ----------------------------
+ * Replaces the reference's code-generation path: generateKernels
+ * (Algorithms/GPU/CodeGeneration/KernelGenerator.cuh:114-176) with
+ * generateUTOrderedKernelCode (UTOrderedKernelCodeGen.cuh:34-583) or
+ * generateNaiveKernelCode (NaiveKernelCodeGen.cuh:34-365), the wrapper's
+ * rebuild-and-rerun protocol (wrapper.cpp:271-344) and the driver
+ * KernelGenSingleGPU (KernelGenSingleGPU.cuh:47-197).  The kernel source is
+ * emitted for the matrix's pattern and exact integer values and compiled
+ * in-process with hiprtc; compiled kernels are cached per (device, hash).
+ * Integer matrices only, 5 <= n <= 63, same exactness bound as
+ * spm_int_exact. */
+#define SPM_CG_REGISTER_EFFICIENT 1  /* rows/columns in IO::UTOrder order
+                                        (REGEFFICIENTCODEGENERATION); without
+                                        it the input order (NAIVECODEGENERATION) */
+int spm_sparse_codegen(const int64_t* a, int n, int options, int device, void* stream, spm_result* out);
+/* The generated source, NUL-terminated, into buf (needed = full size + 1). */
+int spm_codegen_source(const int64_t* a, int n, int options, char* buf, size_t buflen, size_t* needed);
+/* hiprtc-compiles the generated kernel without loading it (no device
+ * needed): compile time and code-object size. */
+int spm_codegen_compile_only(const int64_t* a, int n, int options, double* seconds, size_t* code_bytes);
+/* IO::UTOrder (IO/IO.h:926-1002) on the pattern of `a`: the permuted matrix
+ * is b[row_perm[i]][col_perm[j]] = a[i][j] (IO::applyPermutations,
+ * IO/IO.h:1095-1112).  Host only. */
+int spm_ut_order(const double* a, int n, int* row_perm, int* col_perm);
+
 /* Device-resident plans: the matrix is uploaded once, then spm_plan_run
  * computes with inputs already in HBM (what the benchmark times). */
 typedef struct spm_plan spm_plan;
 int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_plan** plan);
 int spm_plan_dense_c128(const double* a, int n, int precision, int device, spm_plan** plan);
 int spm_plan_int(const int64_t* a, int n, int device, spm_plan** plan);
+/* Compiles (or takes from the cache) the generated kernel for `a`. */
+int spm_plan_codegen(const int64_t* a, int n, int options, int device, spm_plan** plan);
+int spm_plan_codegen_info(const spm_plan* plan, double* compile_seconds, int* cached, uint64_t* hash);
 int spm_plan_run(spm_plan* plan, void* stream, spm_result* out);
 int spm_plan_destroy(spm_plan* plan);
 /* Launch geometry of a plan: grid blocks, block threads, L (Gray steps per

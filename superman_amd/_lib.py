@@ -25,6 +25,8 @@ SPM_EAGAIN = -8
 SPM_ETIMEDOUT = -9
 SPM_EFAULTINJ = -10
 
+SPM_CG_REGISTER_EFFICIENT = 1
+
 SPM_PREC_PLAIN = 0
 SPM_PREC_COMPENSATED = 1
 
@@ -71,6 +73,15 @@ PROTOTYPES = {
     "spm_dense_c128": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_int_exact": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_plan_int": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "spm_sparse_codegen": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
+    "spm_codegen_source": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t)]),
+    "spm_codegen_compile_only": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                                ctypes.POINTER(ctypes.c_size_t)]),
+    "spm_ut_order": (ctypes.c_int, [_D, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "spm_plan_codegen": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "spm_plan_codegen_info": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_uint64)]),
     "spm_plan_dense_f64": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
     "spm_plan_dense_c128": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
     "spm_plan_run": (ctypes.c_int, [_VP, _VP, _R]),

@@ -200,3 +200,78 @@ def device_info(device: int = 0) -> dict:
     name = ctypes.create_string_buffer(256)
     check(lib().spm_device_info(device, ctypes.byref(cu), ctypes.byref(clk), name, 256), "spm_device_info")
     return {"compute_units": cu.value, "clock_khz": clk.value, "name": name.value.decode()}
+
+
+# ---------------------------------------------------------------- SpaRyser codegen
+def _int_square(a) -> np.ndarray:
+    a = np.asarray(a)
+    if a.dtype.kind == "f" and not np.all(a == np.round(a)):
+        raise ValueError("generated SpaRyser kernels need integer entries")
+    return _square(a, np.int64)
+
+
+def sparse_codegen(a, register_efficient: bool = True, device: int = 0, stream: int | None = None) -> Permanent:
+    """Exact permanent through a kernel generated for a's pattern (hiprtc)."""
+    a = _int_square(a)
+    r = SpmResult()
+    opt = _lib.SPM_CG_REGISTER_EFFICIENT if register_efficient else 0
+    check(lib().spm_sparse_codegen(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.shape[0], opt, device, stream,
+                                   ctypes.byref(r)), "spm_sparse_codegen")
+    return _to_perm(r, False)
+
+
+def codegen_source(a, register_efficient: bool = True) -> str:
+    a = _int_square(a)
+    opt = _lib.SPM_CG_REGISTER_EFFICIENT if register_efficient else 0
+    need = ctypes.c_size_t()
+    p = a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    check(lib().spm_codegen_source(p, a.shape[0], opt, None, 0, ctypes.byref(need)), "spm_codegen_source")
+    buf = ctypes.create_string_buffer(need.value)
+    check(lib().spm_codegen_source(p, a.shape[0], opt, buf, need.value, None), "spm_codegen_source")
+    return buf.value.decode()
+
+
+def codegen_compile_only(a, register_efficient: bool = True) -> dict:
+    a = _int_square(a)
+    opt = _lib.SPM_CG_REGISTER_EFFICIENT if register_efficient else 0
+    sec, nbytes = ctypes.c_double(), ctypes.c_size_t()
+    check(lib().spm_codegen_compile_only(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.shape[0], opt,
+                                         ctypes.byref(sec), ctypes.byref(nbytes)), "spm_codegen_compile_only")
+    return {"seconds": sec.value, "code_bytes": nbytes.value}
+
+
+def ut_order(a) -> tuple[np.ndarray, np.ndarray]:
+    """IO::UTOrder on a's pattern: b[row_perm[i], col_perm[j]] = a[i, j]."""
+    a = _square(np.asarray(a, dtype=np.float64), np.float64)
+    n = a.shape[0]
+    rp = np.zeros(n, dtype=np.int32)
+    cp = np.zeros(n, dtype=np.int32)
+    check(lib().spm_ut_order(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                             rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             cp.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "spm_ut_order")
+    return rp, cp
+
+
+def apply_permutations(a, row_perm, col_perm) -> np.ndarray:
+    a = np.asarray(a)
+    b = np.empty_like(a)
+    b[np.ix_(row_perm, col_perm)] = a
+    return b
+
+
+class CodegenPlan(DensePlan):
+    """A generated SpaRyser kernel for one integer matrix (compiled once)."""
+
+    def __init__(self, a, register_efficient: bool = True, device: int = 0):
+        a = _int_square(a)
+        self.complex = False
+        self._h = ctypes.c_void_p()
+        opt = _lib.SPM_CG_REGISTER_EFFICIENT if register_efficient else 0
+        check(lib().spm_plan_codegen(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.shape[0], opt, device,
+                                     ctypes.byref(self._h)), "spm_plan_codegen")
+        self.n = a.shape[0]
+
+    def codegen_info(self) -> dict:
+        sec, cached, h = ctypes.c_double(), ctypes.c_int(), ctypes.c_uint64()
+        check(lib().spm_plan_codegen_info(self._h, ctypes.byref(sec), ctypes.byref(cached), ctypes.byref(h)))
+        return {"compile_seconds": sec.value, "cached": bool(cached.value), "hash": h.value}

@@ -37,6 +37,7 @@ SOURCES = [
     ("dense_c128.hip", []),
     ("int_dense.hip", []),
     ("capi.cpp", []),
+    ("codegen.cpp", []),
 ]
 LINK_LIBS = ["-lhiprtc", "-lpthread"]
 

@@ -126,7 +126,7 @@ using namespace spm;
 // Plans
 // ---------------------------------------------------------------------------
 struct spm_plan {
-    int kind = 0;  // 1 = dense f64, 2 = dense c128, 3 = exact integer
+    int kind = 0;  // 1 = dense f64, 2 = dense c128, 3 = exact integer, 4 = generated SpaRyser
     int n = 0;
     int precision = 0;
     int device = 0;
@@ -145,6 +145,10 @@ struct spm_plan {
     double* h_out = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string kname;
+    hipFunction_t fn = nullptr;  // generated kernel (kind 4), owned by the module cache
+    uint64_t hash = 0;
+    double compile_s = 0;
+    bool cached = false;
 };
 
 static void plan_free(spm_plan* p)
@@ -255,16 +259,18 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     return SPM_OK;
 }
 
-// Exact integer plans (kind 3): y = 2x in int32, T mod 2^128.
-static int make_int_plan(const int64_t* a, int n, int device, spm_plan** out)
+// Integer entry checks shared by the exact kinds: |a| <= 2^30 and absolute
+// row sums < 2^30 (so |y| fits int32), y0 = y(empty set), and log2 of the
+// bound on |T| = 2^(n-1) |perm|: Bregman-Minc for 0/1 matrices, the product
+// of absolute row sums otherwise.  The mod-2^128 sum is T itself only when
+// |T| < 2^127 (checked with margin for the log arithmetic).
+static int int_prepare(const int64_t* a, int n, std::vector<int>& y0, double* log2_bound)
 {
-    if (!out) return SPM_EINVAL;
-    *out = nullptr;
     if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
-    // |y_i| <= 2 sum_j |a_ij| must fit int32.
-    std::vector<int> y0(n);
+    y0.assign(n, 0);
     bool binary = true;
-    double lb = n - 1;  // log2 bound on |T| = 2^(n-1) |perm|
+    for (int e = 0; e < n * n; ++e) binary = binary && (a[e] == 0 || a[e] == 1);
+    double lb = n - 1;
     for (int i = 0; i < n; ++i) {
         int64_t rs = 0, ra = 0;
         for (int j = 0; j < n; ++j) {
@@ -272,16 +278,25 @@ static int make_int_plan(const int64_t* a, int n, int device, spm_plan** out)
             if (v > (int64_t(1) << 30) || v < -(int64_t(1) << 30)) return SPM_ERANGE;
             rs += v;
             ra += v < 0 ? -v : v;
-            binary = binary && (v == 0 || v == 1);
         }
         if (ra >= (int64_t(1) << 30)) return SPM_ERANGE;
         y0[i] = (int)(2 * a[i * n + n - 1] - rs);
         if (ra == 0) lb = -1e300;  // a zero row: perm = 0
         else lb += binary ? std::lgamma((double)ra + 1.0) / ((double)ra * std::log(2.0)) : std::log2((double)ra);
     }
-    // Bregman-Minc (0/1) or row-sum-product bound; the mod-2^128 sum is T
-    // itself only when |T| < 2^127 (with margin for the log arithmetic).
-    if (lb >= 126.0) return SPM_ERANGE;
+    *log2_bound = lb;
+    return lb >= 126.0 ? SPM_ERANGE : SPM_OK;
+}
+
+// Exact integer plans (kind 3): y = 2x in int32, T mod 2^128.
+static int make_int_plan(const int64_t* a, int n, int device, spm_plan** out)
+{
+    if (!out) return SPM_EINVAL;
+    *out = nullptr;
+    std::vector<int> y0;
+    double lb = 0;
+    int rc = int_prepare(a, n, y0, &lb);
+    if (rc != SPM_OK) return rc;
     int cus = 0;
     int st = device_cus(device, &cus);
     if (st != SPM_OK) return st;
@@ -344,6 +359,122 @@ extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int dev
 extern "C" int spm_plan_int(const int64_t* a, int n, int device, spm_plan** out)
 {
     return make_int_plan(a, n, device, out);
+}
+
+// Generated SpaRyser plans (kind 4).
+static std::vector<int64_t> codegen_matrix(const int64_t* a, int n, int options)
+{
+    if (options & SPM_CG_REGISTER_EFFICIENT) {
+        std::vector<int> rp, cp;
+        ut_order(a, n, rp, cp);
+        return apply_perm(a, n, rp, cp);
+    }
+    return std::vector<int64_t>(a, a + (size_t)n * n);
+}
+
+static std::string codegen_name(uint64_t h)
+{
+    char nm[40];
+    std::snprintf(nm, sizeof nm, "spm_gen_%016llx", (unsigned long long)h);
+    return nm;
+}
+
+extern "C" int spm_codegen_source(const int64_t* a, int n, int options, char* buf, size_t buflen, size_t* needed)
+{
+    if (!a || n < kCodegenMinN || n > kMaxN) return SPM_EINVAL;
+    std::vector<int> y0;
+    double lb = 0;
+    const std::vector<int64_t> b = codegen_matrix(a, n, options);
+    int rc = int_prepare(b.data(), n, y0, &lb);
+    if (rc != SPM_OK && rc != SPM_ERANGE) return rc;
+    const std::string src = emit_int_kernel(b, n, codegen_name(pattern_hash(b.data(), n, options)));
+    if (needed) *needed = src.size() + 1;
+    if (buf && buflen > 0) {
+        const size_t k = std::min(buflen - 1, src.size());
+        std::memcpy(buf, src.data(), k);
+        buf[k] = 0;
+    }
+    return SPM_OK;
+}
+
+extern "C" int spm_codegen_compile_only(const int64_t* a, int n, int options, double* seconds, size_t* code_bytes)
+{
+    if (!a || n < kCodegenMinN || n > kMaxN) return SPM_EINVAL;
+    const std::vector<int64_t> b = codegen_matrix(a, n, options);
+    const uint64_t h = pattern_hash(b.data(), n, options);
+    return codegen_compile_only(emit_int_kernel(b, n, codegen_name(h)), codegen_name(h), seconds, code_bytes);
+}
+
+extern "C" int spm_ut_order(const double* a, int n, int* row_perm, int* col_perm)
+{
+    if (!a || !row_perm || !col_perm || n < 1 || n > 4096) return SPM_EINVAL;
+    std::vector<unsigned char> nz((size_t)n * n);
+    for (size_t e = 0; e < nz.size(); ++e) nz[e] = a[e] != 0;
+    std::vector<int> rp, cp;
+    ut_order_mask(nz.data(), n, rp, cp);
+    std::memcpy(row_perm, rp.data(), sizeof(int) * n);
+    std::memcpy(col_perm, cp.data(), sizeof(int) * n);
+    return SPM_OK;
+}
+
+extern "C" int spm_plan_codegen(const int64_t* a, int n, int options, int device, spm_plan** out)
+{
+    if (!out) return SPM_EINVAL;
+    *out = nullptr;
+    if (!a || n < kCodegenMinN || n > kMaxN) return SPM_EINVAL;
+    const std::vector<int64_t> b = codegen_matrix(a, n, options);
+    std::vector<int> y0;
+    double lb = 0;
+    int rc = int_prepare(b.data(), n, y0, &lb);
+    if (rc != SPM_OK) return rc;
+    int cus = 0;
+    rc = device_cus(device, &cus);
+    if (rc != SPM_OK) return rc;
+    DeviceGuard guard(device);
+    if (!guard.ok()) return SPM_EHIP;
+
+    spm_plan* p = new spm_plan;
+    p->kind = 4;
+    p->n = n;
+    p->precision = -1;
+    p->device = device;
+    p->log2_bound = lb;
+    p->rec = 2;
+    p->hash = pattern_hash(b.data(), n, options);
+    p->kname = codegen_name(p->hash);
+    rc = codegen_get_kernel(device, p->hash, emit_int_kernel(b, n, p->kname), p->kname, &p->fn, &p->compile_s,
+                            &p->cached);
+    if (rc != SPM_OK) {
+        plan_free(p);
+        return rc;
+    }
+    int bpc = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, p->fn, kDenseBlock, 0) != hipSuccess || bpc <= 0) {
+        plan_free(p);
+        return SPM_EHIP;
+    }
+    const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
+    gray_geometry(n, codegen_unroll_bits(), waves, &p->L, &p->chunks, &p->tasks);
+    const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
+    p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
+    const double est_s = std::ldexp(1.0, n - 1) * 16.0 / 3.9e13;
+    p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
+    SPM_TRY(hipMalloc(&p->d_partial, sizeof(unsigned long long) * 2 * p->grid));
+    SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 6));
+    SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 6, hipHostMallocDefault));
+    SPM_TRY(hipEventCreate(&p->ev0));
+    SPM_TRY(hipEventCreate(&p->ev1));
+    *out = p;
+    return SPM_OK;
+}
+
+extern "C" int spm_plan_codegen_info(const spm_plan* p, double* compile_seconds, int* cached, uint64_t* hash)
+{
+    if (!p || p->kind != 4) return SPM_EINVAL;
+    if (compile_seconds) *compile_seconds = p->compile_s;
+    if (cached) *cached = p->cached ? 1 : 0;
+    if (hash) *hash = p->hash;
+    return SPM_OK;
 }
 
 extern "C" int spm_plan_dense_c128(const double* a, int n, int precision, int device, spm_plan** out)
@@ -439,8 +570,15 @@ static int int_run(spm_plan* p, hipStream_t st, spm_result* r)
         uint64_t tb = per * s, te = std::min<uint64_t>(p->tasks, per * (s + 1));
         if (tb >= te) break;
         int acc = s > 0 ? 1 : 0;
-        void* args[] = {&p->d_M, &p->d_x0, &p->d_partial, &tb, &te, &p->chunks, &p->L, &acc};
-        if (hipLaunchKernel(p->kern, dim3(p->grid), dim3(kDenseBlock), args, 0, st) != hipSuccess) return SPM_EHIP;
+        if (p->kind == 4) {
+            void* args[] = {&p->d_partial, &tb, &te, &p->chunks, &p->L, &acc};
+            if (hipModuleLaunchKernel(p->fn, p->grid, 1, 1, kDenseBlock, 1, 1, 0, st, args, nullptr) != hipSuccess)
+                return SPM_EHIP;
+        } else {
+            void* args[] = {&p->d_M, &p->d_x0, &p->d_partial, &tb, &te, &p->chunks, &p->L, &acc};
+            if (hipLaunchKernel(p->kern, dim3(p->grid), dim3(kDenseBlock), args, 0, st) != hipSuccess)
+                return SPM_EHIP;
+        }
         ++launches;
     }
     {
@@ -472,7 +610,7 @@ extern "C" int spm_plan_run(spm_plan* p, void* stream, spm_result* out)
 {
     if (!p || !out) return SPM_EINVAL;
     if (p->kind == 1 || p->kind == 2) return dense_run(p, (hipStream_t)stream, out);
-    if (p->kind == 3) return int_run(p, (hipStream_t)stream, out);
+    if (p->kind == 3 || p->kind == 4) return int_run(p, (hipStream_t)stream, out);
     return SPM_EINVAL;
 }
 
@@ -521,6 +659,24 @@ extern "C" int spm_dense_f64(const double* a, int n, int precision, int device, 
 extern "C" int spm_dense_c128(const double* a, int n, int precision, int device, void* stream, spm_result* out)
 {
     return dense_oneshot(2, a, n, precision, device, stream, out);
+}
+
+extern "C" int spm_sparse_codegen(const int64_t* a, int n, int options, int device, void* stream, spm_result* out)
+{
+    if (!out) return SPM_EINVAL;
+    const double t0 = now_seconds();
+    spm_plan* p = nullptr;
+    int st = spm_plan_codegen(a, n, options, device, &p);
+    if (st != SPM_OK) {
+        std::memset(out, 0, sizeof *out);
+        out->status = st;
+        return st;
+    }
+    st = int_run(p, (hipStream_t)stream, out);
+    plan_free(p);
+    if (st == SPM_OK) out->seconds = now_seconds() - t0;
+    else out->status = st;
+    return st;
 }
 
 extern "C" int spm_int_exact(const int64_t* a, int n, int device, void* stream, spm_result* out)

@@ -1,0 +1,361 @@
+// codegen.cpp -- SpaRyser: per-pattern kernels generated and compiled
+// in-process with hiprtc.
+//
+// Replaces the reference's code generation path
+// (Algorithms/GPU/CodeGeneration/KernelGenerator.cuh:114-176,
+// UTOrderedKernelCodeGen.cuh:34-583, NaiveKernelCodeGen.cuh:34-365) and its
+// driver KernelGenSingleGPU (KernelGenSingleGPU.cuh:47-197).  The reference
+// writes generated CUDA into its source tree, exits, has a wrapper rebuild
+// the whole program, and re-runs it (wrapper.cpp:271-344); here the source
+// is compiled by hiprtc inside the call and the code object is cached by a
+// hash of the pattern and values.
+//
+// What is generated, for a matrix whose entries are integers:
+//   * the rows after IO::UTOrder (IO/IO.h:926-1002) for the register-efficient
+//     variant (column order = ascending remaining in-degree, so the columns
+//     flipped most often -- column 0 every other step -- are the sparsest),
+//     or the input order for the naive variant;
+//   * y = 2x kept entirely in VGPRs (at most 63 int32 per lane: the
+//     reference's register-area cost model, UTOrderedKernelCodeGen.cuh:68-132,
+//     trades registers against global-memory x for CUDA's 255-register
+//     limit, which the 512-entry gfx950 file does not need);
+//   * for every column c and direction, the exact row updates
+//     y_r += +-2 a_rc as integer literals (the reference prints FP literals
+//     with std::to_string -- 6 decimals -- which is lossy for general values),
+//     with the zero-row count of each lane kept incrementally;
+//   * the same wave-uniform Gray walk as the dense kernels: lane chunks of
+//     2^L steps, blocks of 2^U steps whose first flip is a runtime column
+//     (a uniform switch) and whose other flips are compile-time columns;
+//   * the term prod_r y_r (128-bit) only on lanes whose zero count is 0, and
+//     128-bit accumulation (mod 2^128; the host proves |T| < 2^127).
+#include "superman_amd.h"
+
+#include "spm_host.h"
+#include "spm_kernels.h"
+
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace spm {
+
+// ---------------------------------------------------------------------------
+// IO::UTOrder (IO/IO.h:926-1002): repeatedly take the first column of least
+// remaining in-degree; rows are numbered in the order they are first touched
+// and, once numbered, no longer count towards the in-degrees.
+// ---------------------------------------------------------------------------
+void ut_order_mask(const unsigned char* nz, int n, std::vector<int>& rowPerm, std::vector<int>& colPerm)
+{
+    rowPerm.assign(n, 0);
+    colPerm.assign(n, 0);
+    for (int i = 0; i < n; ++i) rowPerm[i] = colPerm[i] = i;
+    const unsigned kDone = 0xffffffffu;
+    std::vector<unsigned> indeg(n, 0);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) indeg[j] += nz[i * n + j] != 0;
+    std::vector<bool> marked(n, false);
+    int nextRow = 0;
+    for (int step = 0; step < n; ++step) {
+        unsigned mind = kDone;
+        int chosen = 0;
+        for (int j = 0; j < n; ++j)
+            if (indeg[j] < mind) {
+                mind = indeg[j];
+                chosen = j;
+            }
+        colPerm[chosen] = step;
+        indeg[chosen] = kDone;
+        for (int i = 0; i < n; ++i) {
+            if (nz[i * n + chosen] == 0 || marked[i]) continue;
+            marked[i] = true;
+            rowPerm[i] = nextRow++;
+            for (int j = 0; j < n; ++j)
+                if (indeg[j] != kDone && nz[i * n + j] != 0) --indeg[j];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        if (!marked[i]) rowPerm[i] = nextRow++;
+}
+
+void ut_order(const int64_t* a, int n, std::vector<int>& rowPerm, std::vector<int>& colPerm)
+{
+    std::vector<unsigned char> nz((size_t)n * n);
+    for (size_t e = 0; e < nz.size(); ++e) nz[e] = a[e] != 0;
+    ut_order_mask(nz.data(), n, rowPerm, colPerm);
+}
+
+// newMat[rowPerm[i]][colPerm[j]] = mat[i][j] (IO::applyPermutations, IO/IO.h:1095-1112)
+std::vector<int64_t> apply_perm(const int64_t* a, int n, const std::vector<int>& rp, const std::vector<int>& cp)
+{
+    std::vector<int64_t> b((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) b[(size_t)rp[i] * n + cp[j]] = a[i * n + j];
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// Source emitter (integer entries).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kGenU = 3;  // log2 of the steps per generated block
+
+std::string lit(int64_t v)
+{
+    return std::to_string(v);  // integers are exact in decimal
+}
+
+// y_r += D, keeping Z = number of zero rows.
+void emit_flip_const(std::ostringstream& o, const std::vector<int64_t>& b, int n, int c, int sgn,
+                     const char* ind)
+{
+    for (int r = 0; r < n; ++r) {
+        const int64_t d = 2 * b[(size_t)r * n + c] * sgn;
+        if (d == 0) continue;
+        o << ind << "y" << r << " += " << lit(d) << "; Z += (y" << r << " == 0) - (y" << r << " == " << lit(d)
+          << ");\n";
+    }
+}
+
+// per-lane direction: D = (d ^ lm) - lm with lm = 0 (add) or -1 (remove)
+void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& b, int n, int c, const char* ind)
+{
+    for (int r = 0; r < n; ++r) {
+        const int64_t d = 2 * b[(size_t)r * n + c];
+        if (d == 0) continue;
+        o << ind << "{ const int D = (" << lit(d) << " ^ lm) - lm; y" << r << " += D; Z += (y" << r
+          << " == 0) - (y" << r << " == D); }\n";
+    }
+}
+
+void emit_term(std::ostringstream& o, int n, bool negative, const char* ind)
+{
+    o << ind << "if (Z == 0) {\n";
+    o << ind << "    unsigned __int128 p = (unsigned __int128)(__int128)y0;\n";
+    int r = 1;
+    for (; r + 1 < n; r += 2)
+        o << ind << "    p *= (unsigned __int128)(__int128)((long long)y" << r << " * (long long)y" << r + 1 << ");\n";
+    if (r < n) o << ind << "    p *= (unsigned __int128)(__int128)y" << r << ";\n";
+    o << ind << "    acc " << (negative ? "-" : "+") << "= p;\n";
+    o << ind << "}\n";
+}
+
+}  // namespace
+
+std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::string& name)
+{
+    const int U = kGenU;
+    std::vector<int64_t> y0(n);
+    for (int i = 0; i < n; ++i) {
+        int64_t rs = 0;
+        for (int j = 0; j < n; ++j) rs += b[(size_t)i * n + j];
+        y0[i] = 2 * b[(size_t)i * n + n - 1] - rs;
+    }
+    std::ostringstream o;
+    o << "// generated: SpaRyser kernel for one " << n << "x" << n << " integer pattern\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(256)\n";
+    o << name << "(unsigned long long* __restrict__ partial, unsigned long long task_begin,\n"
+      << "    unsigned long long task_end, unsigned long long num_chunks, int L, int accumulate)\n{\n";
+    o << "    __shared__ unsigned long long red[8];\n"
+         "    const unsigned lane = threadIdx.x & 63u;\n"
+         "    const unsigned long long wave = (unsigned long long)blockIdx.x * 4 + (threadIdx.x >> 6);\n"
+         "    const unsigned long long nwave = (unsigned long long)gridDim.x * 4;\n"
+         "    const bool idle = (unsigned long long)lane >= num_chunks;\n"
+         "    unsigned __int128 acc = 0;\n"
+         "    for (unsigned long long task = task_begin + wave; task < task_end; task += nwave) {\n"
+         "        const unsigned long long h = task * 64 + lane;\n";
+    o << "        int ";
+    for (int i = 0; i < n; ++i) o << "y" << i << " = " << lit(y0[i]) << (i + 1 < n ? ", " : ";\n");
+    o << "        const unsigned long long k0 = h << L;\n"
+         "        const unsigned long long g0 = k0 ^ (k0 >> 1);\n";
+    // chunk-start rebuild: columns j >= L-1 whose bit is set in G(h 2^L)
+    for (int j = 0; j < n - 1; ++j) {
+        bool any = false;
+        for (int r = 0; r < n; ++r) any = any || b[(size_t)r * n + j] != 0;
+        if (!any) continue;
+        o << "        { const int mk = -(int)(((g0 >> " << j << ") & 1ull) & (unsigned)(" << j << " >= L - 1));";
+        for (int r = 0; r < n; ++r) {
+            const int64_t d = 2 * b[(size_t)r * n + j];
+            if (d != 0) o << " y" << r << " += " << lit(d) << " & mk;";
+        }
+        o << " }\n";
+    }
+    o << "        int Z = ";
+    for (int i = 0; i < n; ++i) o << "(y" << i << " == 0)" << (i + 1 < n ? " + " : ";\n");
+    o << "        const int lm = -(int)(h & 1ull);  // direction of column L-1: -1 = remove\n";
+    o << "        const unsigned nblk = 1u << (L - " << U << ");\n"
+         "        const unsigned half = nblk >> 1;\n";
+    // steps j = 1 .. 2^U - 1 of block m: compile-time columns ctz(j); the
+    // direction of column U-1 is bit 0 of m
+    auto block_tail = [&](const char* ind) {
+        for (int j = 1; j < (1 << U); ++j) {
+            const int c = __builtin_ctz(j);
+            if (c + 1 < U) {
+                const int sgn = ((j >> (c + 1)) & 1) ? -1 : 1;
+                emit_flip_const(o, b, n, c, sgn, ind);
+            } else {
+                o << ind << "if (m & 1u) {\n";
+                emit_flip_const(o, b, n, c, -1, (std::string(ind) + "    ").c_str());
+                o << ind << "} else {\n";
+                emit_flip_const(o, b, n, c, 1, (std::string(ind) + "    ").c_str());
+                o << ind << "}\n";
+            }
+            emit_term(o, n, (j & 1) != 0, ind);
+        }
+    };
+    o << "        {\n            const unsigned m = 0;\n";
+    emit_term(o, n, false, "            ");
+    block_tail("            ");
+    o << "        }\n";
+    o << "        for (unsigned m = 1; m < nblk; ++m) {\n"
+         "            const int tz = __builtin_ctz(m);\n"
+         "            const int c = tz + " << U << ";\n"
+         "            const int dir = (m == half) ? 0 : (((m >> (tz + 1)) & 1u) ? -1 : 1);\n"
+         "            switch (c) {\n";
+    for (int c = U; c < n - 1; ++c) {
+        o << "            case " << c << ":\n"
+             "                if (dir == 1) {\n";
+        emit_flip_const(o, b, n, c, 1, "                    ");
+        o << "                } else if (dir == -1) {\n";
+        emit_flip_const(o, b, n, c, -1, "                    ");
+        o << "                } else {\n";
+        emit_flip_lane(o, b, n, c, "                    ");
+        o << "                }\n                break;\n";
+    }
+    o << "            default: break;\n            }\n";
+    emit_term(o, n, false, "            ");
+    block_tail("            ");
+    o << "        }\n";
+    o << "    }\n"
+         "    if (idle) acc = 0;\n"
+         "    unsigned long long lo = (unsigned long long)acc, hi = (unsigned long long)(acc >> 64);\n"
+         "    for (int off = 32; off >= 1; off >>= 1) {\n"
+         "        const unsigned long long olo = __shfl_xor(lo, off, 64);\n"
+         "        const unsigned long long ohi = __shfl_xor(hi, off, 64);\n"
+         "        const unsigned long long s = lo + olo;\n"
+         "        hi = hi + ohi + (s < lo ? 1ull : 0ull);\n"
+         "        lo = s;\n"
+         "    }\n"
+         "    const int w = threadIdx.x >> 6;\n"
+         "    if ((threadIdx.x & 63u) == 0) { red[2 * w] = lo; red[2 * w + 1] = hi; }\n"
+         "    __syncthreads();\n"
+         "    if (threadIdx.x == 0) {\n"
+         "        unsigned __int128 t = 0;\n"
+         "        for (int i = 0; i < 4; ++i) t += ((unsigned __int128)red[2 * i + 1] << 64) | red[2 * i];\n"
+         "        unsigned long long* out = partial + 2 * blockIdx.x;\n"
+         "        if (accumulate) t += ((unsigned __int128)out[1] << 64) | out[0];\n"
+         "        out[0] = (unsigned long long)t;\n"
+         "        out[1] = (unsigned long long)(t >> 64);\n"
+         "    }\n"
+         "}\n";
+    return o.str();
+}
+
+int codegen_unroll_bits() { return kGenU; }
+
+// FNV-1a over the pattern, values and generator options.
+uint64_t pattern_hash(const int64_t* b, int n, int options)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+        for (int k = 0; k < 8; ++k) {
+            h ^= (v >> (8 * k)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    };
+    mix((uint64_t)n);
+    mix((uint64_t)options);
+    mix((uint64_t)kGenU);
+    for (int e = 0; e < n * n; ++e) mix((uint64_t)b[e]);
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// hiprtc compile and per-device module cache
+// ---------------------------------------------------------------------------
+namespace {
+struct CachedKernel {
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+    double compile_s = 0;
+};
+std::mutex g_cg_mu;
+std::map<std::pair<int, uint64_t>, CachedKernel> g_cg_cache;
+}  // namespace
+
+int rtc_compile(const std::string& src, const std::string& name, std::vector<char>& code, std::string& log)
+{
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), (name + ".hip").c_str(), 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return SPM_ECOMPILE;
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    log.assign(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    if (rc != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return SPM_ECOMPILE;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return SPM_OK;
+}
+
+int codegen_compile_only(const std::string& src, const std::string& name, double* seconds, size_t* code_bytes)
+{
+    const double t0 = now_seconds();
+    std::vector<char> code;
+    std::string log;
+    const int st = rtc_compile(src, name, code, log);
+    if (st != SPM_OK) std::fprintf(stderr, "superman_amd: hiprtc failed for %s:\n%s\n", name.c_str(), log.c_str());
+    if (seconds) *seconds = now_seconds() - t0;
+    if (code_bytes) *code_bytes = code.size();
+    return st;
+}
+
+int codegen_get_kernel(int device, uint64_t hash, const std::string& src, const std::string& name, hipFunction_t* fn,
+                       double* compile_s, bool* cached)
+{
+    std::lock_guard<std::mutex> lk(g_cg_mu);
+    auto it = g_cg_cache.find({device, hash});
+    if (it != g_cg_cache.end()) {
+        *fn = it->second.fn;
+        *compile_s = it->second.compile_s;
+        *cached = true;
+        return SPM_OK;
+    }
+    const double t0 = now_seconds();
+    std::vector<char> code;
+    std::string log;
+    int st = rtc_compile(src, name, code, log);
+    if (st != SPM_OK) {
+        std::fprintf(stderr, "superman_amd: hiprtc failed for %s:\n%s\n", name.c_str(), log.c_str());
+        return st;
+    }
+    CachedKernel ck;
+    if (hipModuleLoadData(&ck.module, code.data()) != hipSuccess) return SPM_EHIP;
+    if (hipModuleGetFunction(&ck.fn, ck.module, name.c_str()) != hipSuccess) return SPM_EHIP;
+    ck.compile_s = now_seconds() - t0;
+    g_cg_cache[{device, hash}] = ck;
+    *fn = ck.fn;
+    *compile_s = ck.compile_s;
+    *cached = false;
+    return SPM_OK;
+}
+
+}  // namespace spm

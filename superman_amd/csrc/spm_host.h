@@ -3,6 +3,11 @@
 
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
 namespace spm {
 
 constexpr int kWaveLanes = 64;
@@ -32,6 +37,17 @@ inline double complex_err_est(int n, double abs_sum)
     const double u = 1.1102230246251565e-16;
     return 4.0 * n * u * abs_sum;
 }
+
+// codegen.cpp -- SpaRyser per-pattern kernels
+void ut_order_mask(const unsigned char* nz, int n, std::vector<int>& rowPerm, std::vector<int>& colPerm);
+void ut_order(const int64_t* a, int n, std::vector<int>& rowPerm, std::vector<int>& colPerm);
+int codegen_compile_only(const std::string& src, const std::string& name, double* seconds, size_t* code_bytes);
+std::vector<int64_t> apply_perm(const int64_t* a, int n, const std::vector<int>& rp, const std::vector<int>& cp);
+std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::string& name);
+int codegen_unroll_bits();
+uint64_t pattern_hash(const int64_t* b, int n, int options);
+int codegen_get_kernel(int device, uint64_t hash, const std::string& src, const std::string& name, hipFunction_t* fn,
+                       double* compile_s, bool* cached);
 
 class DeviceGuard {
 public:

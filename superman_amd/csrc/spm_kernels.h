@@ -34,4 +34,7 @@ const void* int_dense_kernel_ptr(int n);
 //   args: (const unsigned long long* partial, int nrec, unsigned long long* out /* [2] */)
 const void* reduce_i128_kernel_ptr();
 
+// Generated SpaRyser kernels walk 2^3-step blocks with L > 3.
+constexpr int kCodegenMinN = 5;
+
 }  // namespace spm

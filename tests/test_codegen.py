@@ -1,0 +1,63 @@
+"""SpaRyser code generation checks that need no GPU (CPU only)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_DIR
+from superman_amd import api
+from superman_amd import workloads as W
+
+
+def test_ut_order_matches_reference_generated_kernel():
+    # The reference's checked-in generated kernel was built from bcsstk01
+    # after IO::UTOrder; every column's rows and (6-decimal) values must match.
+    a = np.load(os.path.join(GOLDEN_DIR, "bcsstk01_undirected.npy"), allow_pickle=False)
+    ref = json.load(open(os.path.join(GOLDEN_DIR, "ut_bcsstk01_reference.json")))["columns"]
+    rp, cp = api.ut_order(a)
+    b = api.apply_permutations(a, rp, cp)
+    n = a.shape[0]
+    assert len(ref) == n
+    for j in range(n):
+        mine = sorted([i, "%f" % b[i, j]] for i in range(n) if b[i, j] != 0)
+        assert mine == ref[str(j)], j
+
+
+def test_ut_order_is_sparsest_first_permutation():
+    a = W.sparse01(30, 5)
+    rp, cp = api.ut_order(a)
+    assert sorted(rp) == list(range(30)) and sorted(cp) == list(range(30))
+    b = api.apply_permutations(a, rp, cp)
+    # column 0 of the ordered matrix is a column of least in-degree
+    assert (b[:, 0] != 0).sum() == min((a[:, j] != 0).sum() for j in range(30))
+    # rows are numbered in first-touch order: row r's first nonzero column is
+    # non-decreasing in r
+    first = [int(np.flatnonzero(b[i])[0]) for i in range(30)]
+    assert first == sorted(first)
+
+
+def test_generated_literals_are_exact():
+    a = np.eye(6, dtype=np.int64) * 123456789 + 1
+    src = api.codegen_source(a, register_efficient=False)
+    assert "+= 246913580;" in src or "+= 246913580" in src  # 2 * (123456789 + 1)
+    assert ".0000" not in src  # no rounded decimal literals
+
+
+def test_generated_source_golden_5x5():
+    a = np.array([[1, 1, 0, 0, 1], [0, 1, 1, 0, 0], [1, 0, 1, 1, 0], [0, 1, 0, 1, 1], [1, 0, 1, 0, 1]])
+    src = api.codegen_source(a)
+    golden = os.path.join(GOLDEN_DIR, "codegen_5x5.txt")
+    assert src == open(golden).read()
+
+
+@pytest.mark.parametrize("n,seed", [(5, 1), (12, 2), (40, 40)])
+def test_generated_source_compiles(n, seed):
+    a = W.sparse01(n, seed) if n > 5 else np.ones((5, 5), dtype=np.int64)
+    info = api.codegen_compile_only(a)
+    assert info["code_bytes"] > 1000
+
+
+def test_codegen_rejects_non_integers():
+    with pytest.raises(ValueError):
+        api.codegen_source(np.full((6, 6), 0.5))
