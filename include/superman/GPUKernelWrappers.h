@@ -1,0 +1,224 @@
+// GPUKernelWrappers.h -- drop-in replacements for the reference's single-GPU
+// entry points, calling libsuperman_amd.so through its C ABI:
+//
+//   gpuDPSingleGPU<C,S>   Algorithms/GPU/GPUKernelWrappers.cu:208-268
+//   gpuSPSingleGPU<C,S>   Algorithms/GPU/GPUKernelWrappers.cu:65-113
+//   gpuComputeComplex     Algorithms/Complex/gpuComputeComplex.cu:199-418
+//
+// and the drivers behind them as Permanent<C,S> subclasses (dpSingleGPU.cuh,
+// spSingleGPU.cuh, KernelGenSingleGPU.cuh).  Same signatures and Result
+// semantics (time = whole-call wall seconds, permanent = double); a library
+// status other than SPM_OK becomes std::runtime_error, as the reference's
+// configuration errors are (its CUDA errors exit() instead).  The
+// reference's DecomposePerman preprocessing (DM pruning, degree-1/2
+// compression, degree-3/4 splitting) is not applied: it changes the
+// arithmetic, never the permanent.
+#ifndef SUPERMAN_AMD_GPUKERNELWRAPPERS_H
+#define SUPERMAN_AMD_GPUKERNELWRAPPERS_H
+
+#include "../superman_amd.h"
+#include "Matrix.h"
+#include "Permanent.h"
+#include "Result.h"
+#include "Settings.h"
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace superman_amd {
+
+inline void check(int status, const char* what)
+{
+    if (status != SPM_OK) throw std::runtime_error(std::string(what) + ": " + spm_strerror(status) + "\n");
+}
+
+inline double wall_seconds()
+{
+    using namespace std::chrono;
+    return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Integer-valued within the exact path's range?
+template <class S>
+inline bool integral(const Matrix<S>* m, std::vector<int64_t>& out)
+{
+    const int nn = m->nov * m->nov;
+    out.resize(nn);
+    for (int e = 0; e < nn; ++e) {
+        const double v = (double)m->mat[e];
+        if (!(std::fabs(v) <= 1073741824.0) || v != std::floor(v)) return false;
+        out[e] = (int64_t)v;
+    }
+    return true;
+}
+
+template <class S>
+inline std::vector<double> as_double(const Matrix<S>* m)
+{
+    std::vector<double> a((size_t)m->nov * m->nov);
+    for (size_t e = 0; e < a.size(); ++e) a[e] = (double)m->mat[e];
+    return a;
+}
+
+inline int precision_of(const Settings& s)
+{
+    if (s.calculationPrecision == DD) return SPM_PREC_PLAIN;
+    if (s.calculationPrecision == KAHAN) return SPM_PREC_COMPENSATED;
+    throw std::runtime_error("Arbitrary precision is only available on the CPU.\n");
+}
+
+}  // namespace superman_amd
+
+// Dense kernels (all dense Algorithm values map to the one dense kernel).
+template <class C, class S>
+class dpSingleGPU : public Permanent<C, S>
+{
+public:
+    dpSingleGPU(Matrix<S>* matrix, Settings settings) : Permanent<C, S>(matrix, settings) {}
+    double permanentFunction() override
+    {
+        const std::vector<double> a = superman_amd::as_double(this->m_Matrix);
+        superman_amd::check(spm_dense_f64(a.data(), this->m_Matrix->nov, superman_amd::precision_of(this->m_Settings),
+                                          this->m_Settings.deviceID, nullptr, &last),
+                            "spm_dense_f64");
+        return last.re;
+    }
+    spm_result last{};
+};
+
+// Generic sparse kernels: integer matrices run exactly, others in FP64.
+template <class C, class S>
+class spSingleGPU : public Permanent<C, S>
+{
+public:
+    spSingleGPU(Matrix<S>* matrix, Settings settings) : Permanent<C, S>(matrix, settings) {}
+    double permanentFunction() override
+    {
+        std::vector<int64_t> ai;
+        const int n = this->m_Matrix->nov;
+        if (superman_amd::integral(this->m_Matrix, ai)) {
+            const int st = spm_int_exact(ai.data(), n, this->m_Settings.deviceID, nullptr, &last);
+            if (st == SPM_OK) return last.re;
+            if (st != SPM_ERANGE) superman_amd::check(st, "spm_int_exact");
+        }
+        const std::vector<double> a = superman_amd::as_double(this->m_Matrix);
+        superman_amd::check(spm_dense_f64(a.data(), n, superman_amd::precision_of(this->m_Settings),
+                                          this->m_Settings.deviceID, nullptr, &last),
+                            "spm_dense_f64");
+        return last.re;
+    }
+    spm_result last{};
+};
+
+// Generated per-pattern kernels (register-efficient = UT-ordered, or naive).
+template <class C, class S>
+class KernelGenSingleGPU : public Permanent<C, S>
+{
+public:
+    KernelGenSingleGPU(Matrix<S>* matrix, Settings settings) : Permanent<C, S>(matrix, settings) {}
+    double permanentFunction() override
+    {
+        std::vector<int64_t> ai;
+        const int n = this->m_Matrix->nov;
+        const int opt = this->m_Settings.algorithm == REGEFFICIENTCODEGENERATION ? SPM_CG_REGISTER_EFFICIENT : 0;
+        if (n >= 5 && superman_amd::integral(this->m_Matrix, ai)) {
+            const int st = spm_sparse_codegen(ai.data(), n, opt, this->m_Settings.deviceID, nullptr, &last);
+            if (st == SPM_OK) return last.re;
+            if (st != SPM_ERANGE) superman_amd::check(st, "spm_sparse_codegen");
+        }
+        spSingleGPU<C, S> generic(new Matrix<S>(*this->m_Matrix), this->m_Settings);
+        const double v = generic.computePermanent();
+        last = generic.last;
+        return v;
+    }
+    spm_result last{};
+};
+
+template <class Job, class C, class S>
+inline Result run_job(Matrix<S>* matrix, Settings* settings)
+{
+    const double t0 = superman_amd::wall_seconds();
+    Job job(new Matrix<S>(*matrix), *settings);
+    const double v = job.computePermanent();
+    return Result(superman_amd::wall_seconds() - t0, v);
+}
+
+template <class C, class S>
+Result gpuDPSingleGPU(Matrix<S>* matrix, Settings* settings)
+{
+    switch (settings->algorithm) {
+    case XREGISTERMSHARED:
+    case XREGISTERMGLOBAL:
+    case XSHAREDMGLOBAL:
+    case XSHAREDMSHARED:
+        return run_job<dpSingleGPU<C, S>, C, S>(matrix, settings);
+    default:
+        throw std::runtime_error("Algorithm you have selected is not included in the available GPU algorithms list.\n");
+    }
+}
+
+template <class C, class S>
+Result gpuSPSingleGPU(Matrix<S>* matrix, Settings* settings)
+{
+    switch (settings->algorithm) {
+    case XSHAREDMGLOBAL:
+    case XSHAREDMSHARED:
+    case XGLOBALMGLOBAL:
+    case XGLOBALMSHARED:
+        return run_job<spSingleGPU<C, S>, C, S>(matrix, settings);
+    case NAIVECODEGENERATION:
+    case REGEFFICIENTCODEGENERATION:
+        return run_job<KernelGenSingleGPU<C, S>, C, S>(matrix, settings);
+    case APPROXIMATION:
+        throw std::runtime_error("The approximation algorithm is not part of the exact-permanent library.\n");
+    default:
+        throw std::runtime_error("Algorithm you have selected is not included in the available GPU algorithms list.\n");
+    }
+}
+
+template <class C, class S>
+Result gpuDPMultiGPU(Matrix<S>*, Settings*)
+{
+    throw std::runtime_error("Multi-GPU modes are not part of this single-GPU build.\n");
+}
+
+template <class C, class S>
+Result gpuSPMultiGPU(Matrix<S>*, Settings*)
+{
+    throw std::runtime_error("Multi-GPU modes are not part of this single-GPU build.\n");
+}
+
+// Complex: the reference prints the result and returns nothing
+// (gpuComputeComplex.cu:406-417); gpuComputeComplexResult returns it.
+inline spm_result gpuComputeComplexResult(Matrix<std::complex<double>>* matrix, Settings* settings)
+{
+    spm_result r{};
+    const int prec = settings->calculationPrecision == DD ? SPM_PREC_PLAIN : SPM_PREC_COMPENSATED;
+    superman_amd::check(spm_dense_c128(reinterpret_cast<const double*>(matrix->mat), matrix->nov, prec,
+                                       settings->deviceID, nullptr, &r),
+                        "spm_dense_c128");
+    return r;
+}
+
+inline void gpuComputeComplex(Matrix<std::complex<double>>* matrix, Settings* settings)
+{
+    const spm_result r = gpuComputeComplexResult(matrix, settings);
+    std::stringstream stream;
+    if (std::isnan(r.re) || std::isnan(r.im)) {
+        stream << "Permanent computation has overflowed." << " - Computed in: " << r.seconds << " seconds." << std::endl;
+    } else {
+        stream << "Permanent: (" << std::setprecision(std::numeric_limits<double>::max_digits10) << r.re << ','
+               << r.im << ") - Computed in: " << r.seconds << " seconds." << std::endl;
+    }
+    if (settings->rank == 0) std::cout << stream.str() << std::flush;
+}
+
+#endif
