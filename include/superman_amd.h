@@ -36,7 +36,7 @@ extern "C" {
 #define SPM_ECANCELLED   -7   /* request cancelled before it ran */
 #define SPM_EAGAIN       -8   /* admission refused (budget), try later */
 #define SPM_ETIMEDOUT    -9   /* request exceeded its deadline */
-#define SPM_EFAULTINJ   -10   /* fault injected through spm_inject_fault */
+#define SPM_EFAULTINJ   -10   /* fault injected through spm_batch_inject_fault */
 
 /* ---- precision modes (Settings.h:59-65 `Precision`) ---------------------- */
 #define SPM_PREC_PLAIN        0   /* reference "dd": plain FP64 sums          */
@@ -152,6 +152,108 @@ int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk, uint64_t* 
                       int* unroll_bits);
 /* Name of the permanent kernel a plan launches (for profiler lookups). */
 const char* spm_plan_kernel_name(const spm_plan* plan);
+
+/* ---- batched and served permanents -----------------------------------------
+ * No single counterpart in the reference, which computes one matrix per
+ * driver call (main.cpp:69-115 calls the selected Algorithm once per
+ * matrix/component, each call allocating, launching a chunk loop and
+ * synchronising: dpSingleGPU.cuh:134-213).  Here many matrices of orders
+ * 1..28 share one device arena and one persistent launch fed by a device
+ * work queue; each matrix is split into 64-chunk work items whose chunk size
+ * comes from the engine's tuning table (set by spm_batch_autotune).  A
+ * matrix's result depends only on its data and its chunk size, never on the
+ * other matrices of its batch.
+ *
+ * Kinds: SPM_KIND_REAL (compensated FP64), SPM_KIND_COMPLEX (compensated
+ * complex128, `a` interleaved re/im), SPM_KIND_INT (exact; `a` holds
+ * integer values as doubles, same range rules as spm_int_exact). */
+#define SPM_KIND_REAL     1
+#define SPM_KIND_COMPLEX  2
+#define SPM_KIND_INT      3
+#define SPM_BATCH_MAX_N   28
+
+#define SPM_EXEC_DEVICE   0  /* run on the GPU                                    */
+#define SPM_EXEC_DRYRUN   1  /* no device at all: batches are formed, "run" for a
+                                time proportional to their work and complete with
+                                NaN values; for testing the admission, deadline,
+                                cancellation and fault paths of the serving loop */
+
+typedef struct spm_batch_config {
+    int device;
+    int slots;              /* batches in flight, one HIP stream each (>= 1)       */
+    int max_requests;       /* matrices per batch                                   */
+    int max_items;          /* work items per batch                                 */
+    size_t arena_bytes;     /* device bytes per batch for matrix data               */
+    int max_pending;        /* admission queue length; submit beyond it: SPM_EAGAIN */
+    double max_batch_work;  /* work budget per batch (FP64-op units, 0 = none)      */
+    int use_graph;          /* launch each batch through a captured hipGraph        */
+    int executor;           /* SPM_EXEC_DEVICE or SPM_EXEC_DRYRUN                   */
+} spm_batch_config;
+
+typedef struct spm_batch_tuning {
+    int fused;              /* 1: one kernel walks every kind; 0: one per kind      */
+    int waves_per_cu;       /* persistent grid size (0 = all resident waves)        */
+    int log2_chunk[4];      /* [kind]: target Gray steps per lane chunk (2^L)       */
+} spm_batch_tuning;
+
+typedef struct spm_request {
+    int kind;
+    int n;
+    const double* a;        /* row-major n x n (complex: 2 n^2 doubles)             */
+    uint64_t tag;           /* returned with the completion                         */
+    double deadline_s;      /* relative to submission; 0 = none                     */
+} spm_request;
+
+typedef struct spm_completion {
+    uint64_t ticket, tag;
+    spm_result result;      /* result.status: SPM_OK or the request's error        */
+    double t_submit, t_launch, t_done;  /* engine clock, seconds                    */
+} spm_completion;
+
+typedef struct spm_serve_stats {
+    uint64_t completed;     /* with status SPM_OK                                   */
+    uint64_t failed;
+    uint64_t batches;
+    double wall_s;          /* first arrival to last completion                     */
+    double perms_per_s;     /* completed / wall_s                                   */
+    double p50_s, p99_s, mean_s, max_s;  /* latency: arrival -> result on host      */
+    double mean_batch;      /* matrices per batch                                   */
+    double device_busy;     /* fraction of wall_s with a batch in flight            */
+} spm_serve_stats;
+
+typedef struct spm_batch spm_batch;
+void spm_batch_config_default(spm_batch_config* cfg);
+int spm_batch_create(const spm_batch_config* cfg, spm_batch** out);
+int spm_batch_destroy(spm_batch* b);
+int spm_batch_get_tuning(const spm_batch* b, spm_batch_tuning* t);
+int spm_batch_set_tuning(spm_batch* b, const spm_batch_tuning* t);
+/* Times `sample` (all submitted at once, best of `reps`) under candidate
+ * tunings -- fused or per-kind kernels, persistent grid size, per-kind chunk
+ * size -- and keeps the fastest.  The engine must be idle. */
+int spm_batch_autotune(spm_batch* b, int count, const spm_request* sample, int reps, spm_batch_tuning* best,
+                       double* best_seconds);
+/* Computes `count` permanents (as many batches as admission needs) and
+ * returns when all are done; out[i] belongs to reqs[i].  Per-request errors
+ * are in out[i].status; the call returns the first one (or SPM_OK). */
+int spm_batch_run(spm_batch* b, int count, const spm_request* reqs, spm_result* out);
+/* Serving: submit copies and prepares the matrix and queues it (SPM_EAGAIN
+ * when the queue is full, SPM_ENOMEM when it can never fit a batch);
+ * progress retires finished batches and launches new ones while a slot is
+ * free; poll returns finished requests (it calls progress first). */
+int spm_serve_submit(spm_batch* b, const spm_request* req, uint64_t* ticket);
+int spm_serve_progress(spm_batch* b, int* in_flight);
+int spm_serve_poll(spm_batch* b, spm_completion* out, int max, int* got);
+/* Cancels a queued or running request: it completes with SPM_ECANCELLED. */
+int spm_serve_cancel(spm_batch* b, uint64_t ticket);
+int spm_serve_pending(spm_batch* b, int* queued, int* in_flight);
+/* Replays an arrival schedule (arrival_s[i] seconds after the start, non-
+ * decreasing) through submit/progress/poll on the calling thread; fills
+ * out[i] and latency_s[i] (arrival to completion) and the stream summary. */
+int spm_serve_replay(spm_batch* b, int count, const spm_request* reqs, const double* arrival_s, spm_result* out,
+                     double* latency_s, spm_serve_stats* stats);
+/* The next `launches` batch launches fail on the host without touching the
+ * device; their requests complete with SPM_EFAULTINJ. */
+int spm_batch_inject_fault(spm_batch* b, int launches);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,13 @@ SPM_CG_REGISTER_EFFICIENT = 1
 SPM_PREC_PLAIN = 0
 SPM_PREC_COMPENSATED = 1
 
+SPM_KIND_REAL = 1
+SPM_KIND_COMPLEX = 2
+SPM_KIND_INT = 3
+SPM_BATCH_MAX_N = 28
+SPM_EXEC_DEVICE = 0
+SPM_EXEC_DRYRUN = 1
+
 
 class SpmResult(ctypes.Structure):
     _fields_ = [
@@ -43,6 +50,35 @@ class SpmResult(ctypes.Structure):
         ("subsets", ctypes.c_uint64),
         ("n", ctypes.c_int32), ("launches", ctypes.c_int32),
     ]
+
+
+class SpmBatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int), ("slots", ctypes.c_int), ("max_requests", ctypes.c_int),
+        ("max_items", ctypes.c_int), ("arena_bytes", ctypes.c_size_t), ("max_pending", ctypes.c_int),
+        ("max_batch_work", ctypes.c_double), ("use_graph", ctypes.c_int), ("executor", ctypes.c_int),
+    ]
+
+
+class SpmBatchTuning(ctypes.Structure):
+    _fields_ = [("fused", ctypes.c_int), ("waves_per_cu", ctypes.c_int), ("log2_chunk", ctypes.c_int * 4)]
+
+
+class SpmRequest(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int), ("a", ctypes.POINTER(ctypes.c_double)),
+                ("tag", ctypes.c_uint64), ("deadline_s", ctypes.c_double)]
+
+
+class SpmCompletion(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("result", SpmResult),
+                ("t_submit", ctypes.c_double), ("t_launch", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class SpmServeStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("wall_s", ctypes.c_double), ("perms_per_s", ctypes.c_double), ("p50_s", ctypes.c_double),
+                ("p99_s", ctypes.c_double), ("mean_s", ctypes.c_double), ("max_s", ctypes.c_double),
+                ("mean_batch", ctypes.c_double), ("device_busy", ctypes.c_double)]
 
 
 class SpmError(RuntimeError):
@@ -92,6 +128,22 @@ PROTOTYPES = {
     "spm_gray_geometry": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_int)]),
+    "spm_batch_config_default": (None, [ctypes.POINTER(SpmBatchConfig)]),
+    "spm_batch_create": (ctypes.c_int, [ctypes.POINTER(SpmBatchConfig), ctypes.POINTER(_VP)]),
+    "spm_batch_destroy": (ctypes.c_int, [_VP]),
+    "spm_batch_get_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(SpmBatchTuning)]),
+    "spm_batch_set_tuning": (ctypes.c_int, [_VP, ctypes.POINTER(SpmBatchTuning)]),
+    "spm_batch_autotune": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(SpmRequest), ctypes.c_int,
+                                          ctypes.POINTER(SpmBatchTuning), ctypes.POINTER(ctypes.c_double)]),
+    "spm_batch_run": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(SpmRequest), _R]),
+    "spm_serve_submit": (ctypes.c_int, [_VP, ctypes.POINTER(SpmRequest), ctypes.POINTER(ctypes.c_uint64)]),
+    "spm_serve_progress": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
+    "spm_serve_poll": (ctypes.c_int, [_VP, ctypes.POINTER(SpmCompletion), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "spm_serve_cancel": (ctypes.c_int, [_VP, ctypes.c_uint64]),
+    "spm_serve_pending": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "spm_serve_replay": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(SpmRequest), _D, _R, _D,
+                                        ctypes.POINTER(SpmServeStats)]),
+    "spm_batch_inject_fault": (ctypes.c_int, [_VP, ctypes.c_int]),
 }
 
 

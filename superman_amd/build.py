@@ -36,7 +36,10 @@ SOURCES = [
     ("dense_f64.hip", []),
     ("dense_c128.hip", []),
     ("int_dense.hip", []),
+    ("batch.hip", []),
     ("capi.cpp", []),
+    ("prep.cpp", []),
+    ("batch_engine.cpp", []),
     ("codegen.cpp", []),
 ]
 LINK_LIBS = ["-lhiprtc", "-lpthread"]

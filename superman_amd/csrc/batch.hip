@@ -1,0 +1,243 @@
+// batch.hip -- many permanents (real FP64, complex128, exact integer; orders
+// 1..kBatchMaxN) in one persistent launch fed by a device work queue.
+//
+// The reference runs one matrix per launch sequence (dpSingleGPU.cuh:148-199
+// and its twins: a power-of-two chunk loop with a device synchronisation per
+// launch).  Here a batch of matrices sits in one device arena; each matrix is
+// split into work items of 64 lane chunks (one wavefront's task: lane h of
+// item t walks the aligned Gray range [(64 t + h) 2^L, (64 t + h + 1) 2^L) of
+// that matrix, exactly as in the single-matrix kernels, walk_*.h).  The grid
+// is sized to the resident wave count; every wave pulls the next item index
+// from a device counter (one atomic per item), dispatches on the item's kind
+// and order (wave-uniform) and writes the item's wave-reduced sum to its own
+// record.  A wave leaves when the counter passes the item count, so every
+// wave reaches the exit.  A second kernel reduces each matrix's item records
+// in a fixed order: the result of a matrix depends only on its data and its
+// chunk size 2^L, never on the batch it rides in or on scheduling.
+//
+// Arena layout (byte offsets in the header, all 64-byte aligned):
+//   header | request descriptors | item -> request table | matrix data
+// Records: item_rec[item][8], result_rec[request][8] (doubles):
+//   real    : hi, lo, abs
+//   complex : re hi, re lo, abs, im hi, im lo
+//   integer : T mod 2^128 as (lo, hi) 64-bit words
+#include "spm_batch.h"
+#include "walk_c128.h"
+#include "walk_f64.h"
+#include "walk_int.h"
+
+#include <utility>
+
+namespace spm {
+
+namespace batch {
+
+// The item of this wave.  Only the lane's chunk index h is per-lane; the
+// descriptor fields are passed through readfirstlane so that the compiler
+// keeps them (and the column addresses built from them) in SGPRs.
+struct Item {
+    const uint8_t* M;
+    const uint8_t* x0;
+    uint64_t h;
+    int L;
+};
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+__device__ __forceinline__ uint64_t uniform(uint64_t v)
+{
+    return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
+}
+
+template <int N>
+__device__ __forceinline__ void item_real(const Item& it, double (&o)[8])
+{
+    const double* M = (const double*)it.M;
+    const double* x0 = (const double*)it.x0;
+    double hi, lo, ab;
+    real::walk_f64<N, real::kDenseUnroll<N>, true>(M, x0, it.h, it.L, hi, lo, ab);
+    o[0] = hi;
+    o[1] = lo;
+    o[2] = ab;
+}
+
+template <int N>
+__device__ __forceinline__ void item_complex(const Item& it, double (&o)[8])
+{
+    const double* M = (const double*)it.M;
+    const double* x0 = (const double*)it.x0;
+    double rh = 0.0, rl = 0.0, ih = 0.0, il = 0.0, ab = 0.0;
+    cplx::walk_c128<N, true>(M, x0, it.h, it.L, rh, rl, ih, il, ab);
+    o[0] = rh;
+    o[1] = rl;
+    o[2] = ab;
+    o[3] = ih;
+    o[4] = il;
+}
+
+template <int N>
+__device__ __forceinline__ void item_int(const Item& it, double (&o)[8])
+{
+    const int* M = (const int*)it.M;
+    const int* y0 = (const int*)it.x0;
+    unsigned __int128 acc = 0;
+    ints::walk_int<N>(M, y0, it.h, it.L, acc);
+    o[0] = __longlong_as_double((long long)(unsigned long long)acc);
+    o[1] = __longlong_as_double((long long)(unsigned long long)(acc >> 64));
+}
+
+template <int KIND, int N>
+__device__ __forceinline__ void run_item(const Item& it, double (&o)[8])
+{
+    if constexpr (KIND == kBatchReal) item_real<N>(it, o);
+    else if constexpr (KIND == kBatchComplex) item_complex<N>(it, o);
+    else item_int<N>(it, o);
+}
+
+// Order dispatch: a chain of uniform compares, one inlined walk per order.
+template <int KIND, int... Is>
+__device__ __forceinline__ void dispatch(int n, const Item& it, double (&o)[8], std::integer_sequence<int, Is...>)
+{
+    (void)((n == Is + 1 ? (run_item<KIND, Is + 1>(it, o), true) : false) || ...);
+}
+
+__device__ __forceinline__ void reduce_i128_wave(unsigned long long& lo, unsigned long long& hi)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long olo = __shfl_xor(lo, off, kWave);
+        const unsigned long long ohi = __shfl_xor(hi, off, kWave);
+        const unsigned long long s = lo + olo;
+        hi = hi + ohi + (s < lo ? 1ull : 0ull);
+        lo = s;
+    }
+}
+
+// Combine the lanes' sums of one item (or of one request's items) in a fixed
+// butterfly order; every lane ends with the total.
+__device__ __forceinline__ void wave_combine(int kind, double (&o)[8])
+{
+    if (kind == kBatchInt) {
+        unsigned long long lo = (unsigned long long)__double_as_longlong(o[0]);
+        unsigned long long hi = (unsigned long long)__double_as_longlong(o[1]);
+        reduce_i128_wave(lo, hi);
+        o[0] = __longlong_as_double((long long)lo);
+        o[1] = __longlong_as_double((long long)hi);
+    } else {
+        wave_reduce_dd(o[0], o[1], o[2]);
+        if (kind == kBatchComplex) {
+            double z = 0.0;
+            wave_reduce_dd(o[3], o[4], z);
+        }
+    }
+}
+
+}  // namespace batch
+
+// KINDS: bit mask of the kinds this instantiation walks (1 real, 2 complex,
+// 4 integer).  The fused kernel (7) walks header range 0 with counter 0; a
+// one-kind kernel walks its kind's range with that kind's counter.
+template <int KINDS>
+__global__ void __launch_bounds__(kDenseBlock)
+batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_rec, unsigned int* __restrict__ counter)
+{
+    using namespace batch;
+    const BatchHeader* hd = (const BatchHeader*)arena;
+    constexpr int q = KINDS == 7 ? 0 : KINDS == 1 ? kBatchReal : KINDS == 2 ? kBatchComplex : kBatchInt;
+    const uint32_t begin = uniform(hd->range_begin[q]);
+    const uint32_t end = uniform(hd->range_end[q]);
+    const BatchRequestDesc* reqs = (const BatchRequestDesc*)(arena + hd->desc_off);
+    const uint32_t* item_req = (const uint32_t*)(arena + hd->item_off);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+
+    for (;;) {
+        uint32_t i = 0;
+        if (lane == 0) i = begin + atomicAdd(counter + q, 1u);
+        i = uniform(i);
+        if (i >= end) break;
+        const BatchRequestDesc* d = reqs + uniform(item_req[i]);
+        Item it;
+        it.M = arena + uniform(d->mat_off);
+        it.x0 = arena + uniform(d->x0_off);
+        it.L = (int)uniform((uint32_t)d->L);
+        it.h = (uint64_t)(i - uniform(d->item_base)) * kWave + lane;
+        const uint64_t num_chunks = uniform(d->num_chunks);
+        const int kind = (int)uniform(d->kind);
+        const int n = (int)uniform(d->n);
+        double o[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        constexpr auto orders = std::make_integer_sequence<int, kBatchMaxN>{};
+        if constexpr ((KINDS & 1) != 0) if (kind == kBatchReal) dispatch<kBatchReal>(n, it, o, orders);
+        if constexpr ((KINDS & 2) != 0) if (kind == kBatchComplex) dispatch<kBatchComplex>(n, it, o, orders);
+        if constexpr ((KINDS & 4) != 0) if (kind == kBatchInt) dispatch<kBatchInt>(n, it, o, orders);
+        if (it.h >= num_chunks) {  // lanes past the matrix's last chunk
+#pragma unroll
+            for (int q = 0; q < 8; ++q) o[q] = 0.0;
+        }
+        wave_combine(kind, o);
+        if (lane == 0) {
+            double* rec = item_rec + (size_t)i * kBatchRec;
+#pragma unroll
+            for (int q = 0; q < kBatchRec; ++q) rec[q] = o[q];
+        }
+    }
+}
+
+// One wave per request: the request's item records in a fixed order (lane l
+// takes items l, l+64, ..., then the butterfly).
+__global__ void __launch_bounds__(kDenseBlock)
+batch_finalize_kernel(const uint8_t* __restrict__ arena, const double* __restrict__ item_rec,
+                      double* __restrict__ result_rec)
+{
+    using namespace batch;
+    const BatchHeader* hd = (const BatchHeader*)arena;
+    const uint32_t r = blockIdx.x * (kDenseBlock / kWave) + threadIdx.x / kWave;
+    if (r >= hd->requests) return;
+    const BatchRequestDesc* d = (const BatchRequestDesc*)(arena + hd->desc_off) + r;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const int kind = (int)d->kind;
+    double o[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    unsigned __int128 t = 0;
+    for (uint32_t j = lane; j < d->items; j += kWave) {
+        const double* rec = item_rec + (size_t)(d->item_base + j) * kBatchRec;
+        if (kind == kBatchInt) {
+            t += ((unsigned __int128)(unsigned long long)__double_as_longlong(rec[1]) << 64) |
+                 (unsigned long long)__double_as_longlong(rec[0]);
+        } else {
+            dd_add(o[0], o[1], rec[0], rec[1]);
+            o[2] += rec[2];
+            if (kind == kBatchComplex) dd_add(o[3], o[4], rec[3], rec[4]);
+        }
+    }
+    if (kind == kBatchInt) {
+        o[0] = __longlong_as_double((long long)(unsigned long long)t);
+        o[1] = __longlong_as_double((long long)(unsigned long long)(t >> 64));
+    }
+    wave_combine(kind, o);
+    if (lane == 0) {
+        double* out = result_rec + (size_t)r * kBatchRec;
+#pragma unroll
+        for (int q = 0; q < kBatchRec; ++q) out[q] = o[q];
+    }
+}
+
+// Resets the four work counters (a kernel, so that a captured graph holds no
+// memset node).
+__global__ void batch_reset_kernel(unsigned int* counter)
+{
+    if (threadIdx.x < 4) counter[threadIdx.x] = 0u;
+}
+
+const void* batch_walk_kernel_ptr(int kinds)
+{
+    switch (kinds) {
+    case 7: return reinterpret_cast<const void*>(&batch_walk_kernel<7>);
+    case 1: return reinterpret_cast<const void*>(&batch_walk_kernel<1>);
+    case 2: return reinterpret_cast<const void*>(&batch_walk_kernel<2>);
+    case 4: return reinterpret_cast<const void*>(&batch_walk_kernel<4>);
+    default: return nullptr;
+    }
+}
+const void* batch_finalize_kernel_ptr() { return reinterpret_cast<const void*>(&batch_finalize_kernel); }
+const void* batch_reset_kernel_ptr() { return reinterpret_cast<const void*>(&batch_reset_kernel); }
+
+}  // namespace spm

@@ -1,0 +1,841 @@
+// batch_engine.cpp -- the batch engine and serving loop behind spm_batch_* and
+// spm_serve_* (include/superman_amd.h).
+//
+// An engine owns `slots` batch contexts, each with its own HIP stream, a
+// pinned staging buffer, a device arena (header, request descriptors, item
+// table, matrix data), item and result records, the work counters and,
+// optionally, a captured hipGraph of the batch's launch sequence
+//   reset counters -> walk kernel(s) -> finalize -> results to pinned host.
+// Requests are prepared on the host at submission (x(empty), +/- planes;
+// integer checks), queue in FIFO order, and are admitted into the next batch
+// while it stays within the slot's request, item, byte and work budgets.
+// A batch is launched as soon as a slot is free, so requests that arrive
+// while earlier batches run are admitted into the next slot and start on
+// the device as the earlier batch's waves drain (continuous batching).
+#include "superman_amd.h"
+
+#include "spm_batch.h"
+#include "spm_host.h"
+#include "spm_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace spm {
+namespace {
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// FP64-op units per Gray subset (work estimates for budgets and ordering).
+double ops_per_subset(int kind, int n)
+{
+    return kind == kBatchComplex ? 8.0 * n : kind == kBatchInt ? 3.0 * n : 2.0 * n;
+}
+
+struct Req {
+    uint64_t ticket = 0, tag = 0;
+    int kind = 0, n = 0;
+    std::vector<uint8_t> data;  // planes, then x0 at x0_rel (64-byte aligned)
+    size_t x0_rel = 0;
+    int L = 0;
+    uint64_t chunks = 0;
+    uint32_t items = 0;
+    double item_cost = 0, work = 0;
+    double t_submit = 0, t_deadline = 0, t_launch = 0;
+    bool cancelled = false;
+    int index = -1;             // position in a run/replay, -1 for served requests
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint8_t* h_stage = nullptr;
+    uint8_t* d_arena = nullptr;
+    double* d_item = nullptr;
+    double* d_res = nullptr;
+    double* h_res = nullptr;
+    unsigned int* d_counter = nullptr;
+    std::vector<Req*> reqs;
+    bool busy = false;
+    bool fault = false;
+    double t_launch = 0, t_ready = 0;
+};
+
+}  // namespace
+}  // namespace spm
+
+using namespace spm;
+
+struct spm_batch {
+    spm_batch_config cfg{};
+    spm_batch_tuning tune{};
+    int cus = 0;
+    int max_wpc[4] = {0, 0, 0, 0};  // resident waves per CU: [0] fused, [kind] one-kind kernel
+    size_t desc_off = 0, item_off = 0, data_off = 0, stage_bytes = 0;
+    std::vector<Slot> slots;
+    std::deque<Req*> pending;
+    std::deque<spm_completion> done;
+    std::unordered_map<uint64_t, Req*> live;
+    uint64_t next_ticket = 1;
+    int fault_launches = 0;
+    uint64_t batches = 0;
+    double busy_since = -1, busy_total = 0;
+    int graphs_valid = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+bool dry(const spm_batch* b) { return b->cfg.executor == SPM_EXEC_DRYRUN; }
+
+// Chunk size for a request: L >= 2 for the pipelined walks (n >= 3); n = 2
+// walks one 2-step chunk per lane (L = 1) and n = 1 its single subset.
+void geometry(const spm_batch* b, Req* r)
+{
+    const int B = r->n - 1;
+    int L;
+    if (r->n == 1) L = 0;
+    else if (r->n == 2) L = 1;
+    else L = std::max(2, std::min({b->tune.log2_chunk[r->kind], B - 6, B}));
+    r->L = L;
+    r->chunks = uint64_t(1) << (B - L);
+    r->items = (uint32_t)((r->chunks + kWaveLanes - 1) / kWaveLanes);
+    const double ops = ops_per_subset(r->kind, r->n);
+    r->item_cost = std::ldexp(1.0, L) * kWaveLanes * ops + double(r->n) * r->n;
+    r->work = std::ldexp(1.0, B) * ops;
+}
+
+int prepare(const spm_request* q, Req* r)
+{
+    if (!q || !q->a) return SPM_EINVAL;
+    if (q->kind != SPM_KIND_REAL && q->kind != SPM_KIND_COMPLEX && q->kind != SPM_KIND_INT) return SPM_EINVAL;
+    const int n = q->n;
+    if (n < 1 || n > kBatchMaxN) return SPM_EINVAL;
+    const int w = q->kind == SPM_KIND_COMPLEX ? 2 : 1;
+    for (int e = 0; e < w * n * n; ++e)
+        if (!std::isfinite(q->a[e])) return SPM_EINVAL;
+    r->kind = q->kind;
+    r->n = n;
+    r->tag = q->tag;
+    if (q->kind == SPM_KIND_INT) {
+        std::vector<int64_t> ai((size_t)n * n);
+        for (int e = 0; e < n * n; ++e) {
+            const double v = q->a[e];
+            if (v != std::floor(v) || std::fabs(v) > 1073741824.0) return SPM_ENOTINT;
+            ai[e] = (int64_t)v;
+        }
+        const size_t planes = sizeof(int) * 2 * (size_t)n * n;
+        r->x0_rel = align_up(planes, kBatchAlign);
+        r->data.assign(r->x0_rel + sizeof(int) * n, 0);
+        double lb = 0;
+        const int st = int_prepare(ai.data(), n, (int*)(r->data.data() + r->x0_rel), &lb);
+        if (st != SPM_OK) return st;
+        int_planes(ai.data(), n, (int*)r->data.data());
+    } else {
+        const size_t planes = sizeof(double) * 2 * (size_t)w * n * n;
+        r->x0_rel = align_up(planes, kBatchAlign);
+        r->data.assign(r->x0_rel + sizeof(double) * w * n, 0);
+        dense_prepare(q->a, n, w, (double*)r->data.data(), (double*)(r->data.data() + r->x0_rel));
+    }
+    return SPM_OK;
+}
+
+void complete(spm_batch* b, Req* r, int status, const double* rec, double t_done, float batch_ms)
+{
+    spm_completion c;
+    std::memset(&c, 0, sizeof c);
+    c.ticket = r->ticket;
+    c.tag = r->tag;
+    c.t_submit = r->t_submit;
+    c.t_launch = r->t_launch;
+    c.t_done = t_done;
+    spm_result& res = c.result;
+    if (status == SPM_OK && r->cancelled) status = SPM_ECANCELLED;
+    if (status == SPM_OK && rec) {
+        if (r->kind == SPM_KIND_INT) {
+            uint64_t w[2];
+            std::memcpy(w, rec, 16);
+            status = int_fill_result(r->n, w[0], w[1], &res);
+        } else {
+            dense_fill_result(r->n, r->kind == SPM_KIND_COMPLEX, true, rec, &res);
+        }
+    } else if (status == SPM_OK) {  // dry run: scheduled, nothing computed
+        res.re = res.im = std::numeric_limits<double>::quiet_NaN();
+        res.n = r->n;
+        res.subsets = uint64_t(1) << (r->n - 1);
+    }
+    res.status = status;
+    res.kernel_ms = batch_ms;
+    res.seconds = t_done - r->t_submit;
+    res.launches = rec ? 1 : 0;
+    b->live.erase(r->ticket);
+    b->done.push_back(c);
+    delete r;
+}
+
+void free_graph(Slot& s)
+{
+    if (s.exec) (void)hipGraphExecDestroy(s.exec);
+    if (s.graph) (void)hipGraphDestroy(s.graph);
+    s.exec = nullptr;
+    s.graph = nullptr;
+}
+
+void free_slot(Slot& s)
+{
+    free_graph(s);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.ev0) (void)hipEventDestroy(s.ev0);
+    if (s.ev1) (void)hipEventDestroy(s.ev1);
+    if (s.h_stage) (void)hipHostFree(s.h_stage);
+    if (s.h_res) (void)hipHostFree(s.h_res);
+    if (s.d_arena) (void)hipFree(s.d_arena);
+    if (s.d_item) (void)hipFree(s.d_item);
+    if (s.d_res) (void)hipFree(s.d_res);
+    if (s.d_counter) (void)hipFree(s.d_counter);
+    s = Slot();
+}
+
+int grid_for(const spm_batch* b, int q)
+{
+    int wpc = b->max_wpc[q];
+    if (b->tune.waves_per_cu > 0) wpc = std::min(wpc, std::max(kDenseBlock / kWaveLanes, b->tune.waves_per_cu));
+    return std::max(1, b->cus * wpc / (kDenseBlock / kWaveLanes));
+}
+
+// The launch sequence of one batch after its arena upload; `nreq` sizes the
+// finalize grid and the result copy (the graph uses the slot maximum).
+hipError_t enqueue(spm_batch* b, Slot& s, int nreq)
+{
+    hipError_t e;
+    {
+        void* args[] = {&s.d_counter};
+        e = hipLaunchKernel(batch_reset_kernel_ptr(), dim3(1), dim3(kWaveLanes), args, 0, s.stream);
+        if (e != hipSuccess) return e;
+    }
+    const int kinds[3] = {1, 2, 4};
+    for (int k = 0; k < (b->tune.fused ? 1 : 3); ++k) {
+        const int mask = b->tune.fused ? 7 : kinds[k];
+        const int q = b->tune.fused ? 0 : k + 1;
+        void* args[] = {&s.d_arena, &s.d_item, &s.d_counter};
+        e = hipLaunchKernel(batch_walk_kernel_ptr(mask), dim3(grid_for(b, q)), dim3(kDenseBlock), args, 0, s.stream);
+        if (e != hipSuccess) return e;
+    }
+    {
+        const int waves = kDenseBlock / kWaveLanes;
+        void* args[] = {&s.d_arena, &s.d_item, &s.d_res};
+        e = hipLaunchKernel(batch_finalize_kernel_ptr(), dim3((nreq + waves - 1) / waves), dim3(kDenseBlock), args, 0,
+                            s.stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipMemcpyAsync(s.h_res, s.d_res, sizeof(double) * kBatchRec * nreq, hipMemcpyDeviceToHost, s.stream);
+}
+
+hipError_t capture(spm_batch* b, Slot& s)
+{
+    free_graph(s);
+    hipError_t e = hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return e;
+    hipError_t e1 = enqueue(b, s, b->cfg.max_requests);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s.stream, &g);
+    if (e1 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return e1;
+    }
+    if (e != hipSuccess) return e;
+    s.graph = g;
+    return hipGraphInstantiate(&s.exec, g, nullptr, nullptr, 0);
+}
+
+// Admission: FIFO with skip-ahead, within the slot's budgets.  Requests past
+// their deadline complete with SPM_ETIMEDOUT here.
+std::vector<Req*> admit(spm_batch* b, double now)
+{
+    std::vector<Req*> out;
+    size_t bytes = 0;
+    uint64_t items = 0;
+    double work = 0;
+    for (auto it = b->pending.begin(); it != b->pending.end() && (int)out.size() < b->cfg.max_requests;) {
+        Req* r = *it;
+        if (r->cancelled) {
+            it = b->pending.erase(it);
+            complete(b, r, SPM_ECANCELLED, nullptr, now, 0.f);
+            continue;
+        }
+        if (r->t_deadline > 0 && now > r->t_deadline) {
+            it = b->pending.erase(it);
+            complete(b, r, SPM_ETIMEDOUT, nullptr, now, 0.f);
+            continue;
+        }
+        geometry(b, r);
+        const size_t rb = align_up(r->data.size(), kBatchAlign);
+        const bool fits = bytes + rb <= b->cfg.arena_bytes && items + r->items <= (uint64_t)b->cfg.max_items &&
+                          (out.empty() || b->cfg.max_batch_work <= 0 || work + r->work <= b->cfg.max_batch_work);
+        if (fits) {
+            out.push_back(r);
+            bytes += rb;
+            items += r->items;
+            work += r->work;
+            it = b->pending.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    return out;
+}
+
+// Writes header, descriptors, item table and data into the slot's staging
+// buffer; returns the bytes to upload.  Item order: fused -- every item by
+// decreasing cost; per kind -- grouped by kind, decreasing cost inside.
+size_t stage(spm_batch* b, Slot& s, std::vector<Req*>& rs)
+{
+    if (b->tune.fused) {
+        std::stable_sort(rs.begin(), rs.end(), [](const Req* x, const Req* y) { return x->item_cost > y->item_cost; });
+    } else {
+        std::stable_sort(rs.begin(), rs.end(), [](const Req* x, const Req* y) {
+            return x->kind != y->kind ? x->kind < y->kind : x->item_cost > y->item_cost;
+        });
+    }
+    uint8_t* h = s.h_stage;
+    BatchHeader hd;
+    std::memset(&hd, 0, sizeof hd);
+    hd.requests = (uint32_t)rs.size();
+    hd.desc_off = b->desc_off;
+    hd.item_off = b->item_off;
+    BatchRequestDesc* desc = (BatchRequestDesc*)(h + b->desc_off);
+    uint32_t* item_req = (uint32_t*)(h + b->item_off);
+    size_t data = b->data_off;
+    uint32_t item = 0;
+    for (int k = 0; k < 4; ++k) hd.range_begin[k] = hd.range_end[k] = 0;
+    for (size_t j = 0; j < rs.size(); ++j) {
+        Req* r = rs[j];
+        if (hd.range_end[r->kind] == 0 && hd.range_begin[r->kind] == 0) hd.range_begin[r->kind] = item;
+        BatchRequestDesc d;
+        std::memset(&d, 0, sizeof d);
+        d.kind = (uint32_t)r->kind;
+        d.n = (uint32_t)r->n;
+        d.L = r->L;
+        d.items = r->items;
+        d.num_chunks = r->chunks;
+        d.mat_off = data;
+        d.x0_off = data + r->x0_rel;
+        d.item_base = item;
+        std::memcpy(desc + j, &d, sizeof d);
+        std::memcpy(h + data, r->data.data(), r->data.size());
+        data += align_up(r->data.size(), kBatchAlign);
+        for (uint32_t t = 0; t < r->items; ++t) item_req[item + t] = (uint32_t)j;
+        item += r->items;
+        hd.range_end[r->kind] = item;
+    }
+    hd.items = item;
+    hd.range_begin[0] = 0;
+    hd.range_end[0] = item;
+    std::memcpy(h, &hd, sizeof hd);
+    return data;
+}
+
+int launch(spm_batch* b, Slot& s, std::vector<Req*>& rs, double now)
+{
+    s.reqs = rs;
+    s.busy = true;
+    s.fault = false;
+    s.t_launch = now;
+    for (Req* r : rs) r->t_launch = now;
+    ++b->batches;
+    if (b->busy_since < 0) b->busy_since = now;
+    if (b->fault_launches > 0) {
+        --b->fault_launches;
+        s.fault = true;
+        return SPM_OK;
+    }
+    if (dry(b)) {
+        double work = 0;
+        for (Req* r : rs) work += r->work;
+        s.t_ready = now + 20e-6 + work / 2e13;
+        return SPM_OK;
+    }
+    const size_t bytes = stage(b, s, rs);
+    s.reqs = rs;  // staged order = record order
+    DeviceGuard g(b->cfg.device);
+    if (!g.ok()) return SPM_EHIP;
+    if (hipEventRecord(s.ev0, s.stream) != hipSuccess) return SPM_EHIP;
+    if (hipMemcpyAsync(s.d_arena, s.h_stage, bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess) return SPM_EHIP;
+    if (b->cfg.use_graph) {
+        if (!s.exec && capture(b, s) != hipSuccess) return SPM_EHIP;
+        if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return SPM_EHIP;
+    } else if (enqueue(b, s, (int)rs.size()) != hipSuccess) {
+        return SPM_EHIP;
+    }
+    if (hipEventRecord(s.ev1, s.stream) != hipSuccess) return SPM_EHIP;
+    return SPM_OK;
+}
+
+void fail_slot(spm_batch* b, Slot& s, int status, double now)
+{
+    for (Req* r : s.reqs) complete(b, r, status, nullptr, now, 0.f);
+    s.reqs.clear();
+    s.busy = false;
+}
+
+// Retires finished slots; returns how many were retired.
+int retire(spm_batch* b, double now)
+{
+    int got = 0;
+    for (Slot& s : b->slots) {
+        if (!s.busy) continue;
+        if (s.fault) {
+            fail_slot(b, s, SPM_EFAULTINJ, now);
+            ++got;
+            continue;
+        }
+        if (dry(b)) {
+            if (now < s.t_ready) continue;
+            for (Req* r : s.reqs) complete(b, r, SPM_OK, nullptr, now, 0.f);
+            s.reqs.clear();
+            s.busy = false;
+            ++got;
+            continue;
+        }
+        const hipError_t e = hipEventQuery(s.ev1);
+        if (e == hipErrorNotReady) continue;
+        if (e != hipSuccess) {
+            fail_slot(b, s, SPM_EHIP, now);
+            ++got;
+            continue;
+        }
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s.ev0, s.ev1) != hipSuccess) ms = -1.f;
+        for (size_t j = 0; j < s.reqs.size(); ++j)
+            complete(b, s.reqs[j], SPM_OK, s.h_res + j * kBatchRec, now, ms);
+        s.reqs.clear();
+        s.busy = false;
+        ++got;
+    }
+    return got;
+}
+
+int progress(spm_batch* b, int* in_flight)
+{
+    double now = now_seconds();
+    retire(b, now);
+    for (Slot& s : b->slots) {
+        if (s.busy || b->pending.empty()) continue;
+        std::vector<Req*> rs = admit(b, now);
+        if (rs.empty()) break;
+        const int st = launch(b, s, rs, now);
+        if (st != SPM_OK) fail_slot(b, s, st, now);
+    }
+    int busy = 0;
+    for (const Slot& s : b->slots) busy += s.busy ? 1 : 0;
+    now = now_seconds();
+    if (busy == 0 && b->busy_since >= 0) {
+        b->busy_total += now - b->busy_since;
+        b->busy_since = -1;
+    }
+    if (in_flight) *in_flight = busy;
+    return SPM_OK;
+}
+
+bool idle(const spm_batch* b)
+{
+    if (!b->pending.empty()) return false;
+    for (const Slot& s : b->slots)
+        if (s.busy) return false;
+    return true;
+}
+
+int submit(spm_batch* b, const spm_request* q, double t_submit, int index, uint64_t* ticket)
+{
+    if ((int)b->pending.size() >= b->cfg.max_pending) return SPM_EAGAIN;
+    std::unique_ptr<Req> r(new Req);
+    int st = prepare(q, r.get());
+    if (st != SPM_OK) return st;
+    geometry(b, r.get());
+    if (align_up(r->data.size(), kBatchAlign) > b->cfg.arena_bytes || r->items > (uint32_t)b->cfg.max_items)
+        return SPM_ENOMEM;
+    r->ticket = b->next_ticket++;
+    r->t_submit = t_submit;
+    r->t_deadline = q->deadline_s > 0 ? t_submit + q->deadline_s : 0;
+    r->index = index;
+    if (ticket) *ticket = r->ticket;
+    b->live[r->ticket] = r.get();
+    b->pending.push_back(r.release());
+    return SPM_OK;
+}
+
+void set_status(spm_result* out, int st)
+{
+    std::memset(out, 0, sizeof *out);
+    out->status = st;
+}
+
+// Drives `count` requests through the engine: all at t = 0 (arrival_s =
+// NULL) or at their arrival times.  Per-request results/latencies by index.
+int drive(spm_batch* b, int count, const spm_request* reqs, const double* arrival_s, spm_result* out,
+          double* latency_s, spm_serve_stats* stats)
+{
+    if (count < 0 || (count > 0 && (!reqs || !out))) return SPM_EINVAL;
+    if (!idle(b) || !b->done.empty()) return SPM_EAGAIN;
+    std::unordered_map<uint64_t, int> index;
+    std::vector<double> lat(count, 0.0);
+    const uint64_t batches0 = b->batches;
+    b->busy_total = 0;
+    b->busy_since = -1;
+    const double t0 = now_seconds();
+    int next = 0, finished = 0, first_err = SPM_OK;
+    double t_last = t0;
+    while (finished < count) {
+        double now = now_seconds();
+        while (next < count && (!arrival_s || t0 + arrival_s[next] <= now)) {
+            uint64_t tk = 0;
+            const double t_arr = arrival_s ? t0 + arrival_s[next] : t0;
+            const int st = submit(b, reqs + next, t_arr, next, &tk);
+            if (st == SPM_EAGAIN) break;  // queue full: retry after progress
+            if (st != SPM_OK) {
+                set_status(out + next, st);
+                lat[next] = 0.0;
+                if (first_err == SPM_OK) first_err = st;
+                ++finished;
+            } else {
+                index[tk] = next;
+            }
+            ++next;
+        }
+        int busy = 0;
+        progress(b, &busy);
+        bool any = false;
+        while (!b->done.empty()) {
+            const spm_completion c = b->done.front();
+            b->done.pop_front();
+            auto it = index.find(c.ticket);
+            if (it == index.end()) continue;
+            const int i = it->second;
+            out[i] = c.result;
+            lat[i] = c.t_done - (arrival_s ? t0 + arrival_s[i] : t0);
+            if (c.result.status != SPM_OK && first_err == SPM_OK) first_err = c.result.status;
+            t_last = std::max(t_last, c.t_done);
+            ++finished;
+            any = true;
+        }
+        if (!any && busy == 0 && b->pending.empty() && next < count && arrival_s) {
+            // idle until the next arrival
+            const double wait = t0 + arrival_s[next] - now_seconds();
+            if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
+        }
+    }
+    if (latency_s)
+        for (int i = 0; i < count; ++i) latency_s[i] = lat[i];
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        std::vector<double> ok;
+        for (int i = 0; i < count; ++i) {
+            if (out[i].status == SPM_OK) ok.push_back(lat[i]);
+            else ++stats->failed;
+        }
+        stats->completed = ok.size();
+        stats->batches = b->batches - batches0;
+        const double first = arrival_s && count > 0 ? t0 + arrival_s[0] : t0;
+        stats->wall_s = t_last - first;
+        stats->perms_per_s = stats->wall_s > 0 ? ok.size() / stats->wall_s : 0.0;
+        if (!ok.empty()) {
+            std::sort(ok.begin(), ok.end());
+            auto pct = [&](double p) {
+                const size_t k = (size_t)std::min<double>(ok.size() - 1, std::ceil(p * ok.size()) - 1);
+                return ok[k];
+            };
+            stats->p50_s = pct(0.50);
+            stats->p99_s = pct(0.99);
+            double s = 0;
+            for (double v : ok) s += v;
+            stats->mean_s = s / ok.size();
+            stats->max_s = ok.back();
+        }
+        stats->mean_batch = stats->batches ? double(count) / stats->batches : 0.0;
+        stats->device_busy = stats->wall_s > 0 ? std::min(1.0, b->busy_total / stats->wall_s) : 0.0;
+    }
+    return first_err;
+}
+
+}  // namespace
+
+extern "C" void spm_batch_config_default(spm_batch_config* c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->device = 0;
+    c->slots = 2;
+    c->max_requests = 1024;
+    c->max_items = 1 << 16;
+    c->arena_bytes = size_t(64) << 20;
+    c->max_pending = 1 << 16;
+    c->max_batch_work = 0;
+    c->use_graph = 1;
+    c->executor = SPM_EXEC_DEVICE;
+}
+
+extern "C" int spm_batch_create(const spm_batch_config* cfg, spm_batch** out)
+{
+    if (!cfg || !out) return SPM_EINVAL;
+    *out = nullptr;
+    if (cfg->slots < 1 || cfg->slots > 8 || cfg->max_requests < 1 || cfg->max_requests > (1 << 20) ||
+        cfg->max_items < 1 || cfg->max_items > (1 << 26) || cfg->arena_bytes < 1024 || cfg->max_pending < 1)
+        return SPM_EINVAL;
+    if (cfg->executor != SPM_EXEC_DEVICE && cfg->executor != SPM_EXEC_DRYRUN) return SPM_EINVAL;
+    std::unique_ptr<spm_batch> b(new spm_batch);
+    b->cfg = *cfg;
+    b->tune.fused = 0;
+    b->tune.waves_per_cu = 0;
+    b->tune.log2_chunk[0] = 0;
+    b->tune.log2_chunk[SPM_KIND_REAL] = 10;
+    b->tune.log2_chunk[SPM_KIND_COMPLEX] = 9;
+    b->tune.log2_chunk[SPM_KIND_INT] = 10;
+    b->desc_off = align_up(sizeof(BatchHeader), kBatchAlign);
+    b->item_off = align_up(b->desc_off + sizeof(BatchRequestDesc) * cfg->max_requests, kBatchAlign);
+    b->data_off = align_up(b->item_off + sizeof(uint32_t) * cfg->max_items, kBatchAlign);
+    b->stage_bytes = b->data_off + align_up(cfg->arena_bytes, kBatchAlign);
+    b->slots.resize(cfg->slots);
+    if (dry(b.get())) {
+        b->cus = 256;
+        for (int q = 0; q < 4; ++q) b->max_wpc[q] = 8;
+        *out = b.release();
+        return SPM_OK;
+    }
+    int st = device_cus(cfg->device, &b->cus);
+    if (st != SPM_OK) return st;
+    DeviceGuard g(cfg->device);
+    if (!g.ok()) return SPM_EHIP;
+    const int masks[4] = {7, 1, 2, 4};
+    for (int q = 0; q < 4; ++q) {
+        int bpc = 0;
+        st = blocks_per_cu(cfg->device, batch_walk_kernel_ptr(masks[q]), kDenseBlock, &bpc);
+        if (st != SPM_OK) return st;
+        b->max_wpc[q] = bpc * (kDenseBlock / kWaveLanes);
+    }
+    for (Slot& s : b->slots) {
+        bool ok = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreate(&s.ev0) == hipSuccess && hipEventCreate(&s.ev1) == hipSuccess;
+        if (!ok) {
+            for (Slot& t : b->slots) free_slot(t);
+            return SPM_EHIP;
+        }
+        ok = hipHostMalloc(&s.h_stage, b->stage_bytes, hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc(&s.h_res, sizeof(double) * kBatchRec * cfg->max_requests, hipHostMallocDefault) ==
+                 hipSuccess &&
+             hipMalloc(&s.d_arena, b->stage_bytes) == hipSuccess &&
+             hipMalloc(&s.d_item, sizeof(double) * kBatchRec * (size_t)cfg->max_items) == hipSuccess &&
+             hipMalloc(&s.d_res, sizeof(double) * kBatchRec * cfg->max_requests) == hipSuccess &&
+             hipMalloc(&s.d_counter, 256) == hipSuccess;
+        if (!ok) {
+            for (Slot& t : b->slots) free_slot(t);
+            return SPM_ENOMEM;
+        }
+    }
+    *out = b.release();
+    return SPM_OK;
+}
+
+extern "C" int spm_batch_destroy(spm_batch* b)
+{
+    if (!b) return SPM_OK;
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        if (!dry(b)) {
+            DeviceGuard g(b->cfg.device);
+            for (Slot& s : b->slots)
+                if (s.stream) (void)hipStreamSynchronize(s.stream);
+            for (Slot& s : b->slots) free_slot(s);
+        }
+        for (Slot& s : b->slots)
+            for (Req* r : s.reqs) delete r;
+        for (Req* r : b->pending) delete r;
+    }
+    delete b;
+    return SPM_OK;
+}
+
+extern "C" int spm_batch_get_tuning(const spm_batch* b, spm_batch_tuning* t)
+{
+    if (!b || !t) return SPM_EINVAL;
+    *t = b->tune;
+    return SPM_OK;
+}
+
+static int set_tuning_locked(spm_batch* b, const spm_batch_tuning* t)
+{
+    if (t->fused != 0 && t->fused != 1) return SPM_EINVAL;
+    if (t->waves_per_cu < 0 || t->waves_per_cu > 64) return SPM_EINVAL;
+    for (int k = 1; k < 4; ++k)
+        if (t->log2_chunk[k] < 2 || t->log2_chunk[k] > 30) return SPM_EINVAL;
+    for (const Slot& s : b->slots)
+        if (s.busy) return SPM_EAGAIN;
+    b->tune = *t;
+    for (Slot& s : b->slots) free_graph(s);  // grids and kernels are baked into the graphs
+    return SPM_OK;
+}
+
+extern "C" int spm_batch_set_tuning(spm_batch* b, const spm_batch_tuning* t)
+{
+    if (!b || !t) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    return set_tuning_locked(b, t);
+}
+
+extern "C" int spm_batch_run(spm_batch* b, int count, const spm_request* reqs, spm_result* out)
+{
+    if (!b) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    return drive(b, count, reqs, nullptr, out, nullptr, nullptr);
+}
+
+extern "C" int spm_batch_autotune(spm_batch* b, int count, const spm_request* sample, int reps, spm_batch_tuning* best,
+                                  double* best_seconds)
+{
+    if (!b || count < 1 || !sample || reps < 1) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (!idle(b)) return SPM_EAGAIN;
+    std::vector<spm_result> out(count);
+    auto time_it = [&](const spm_batch_tuning& t, int cnt, const spm_request* rq, double* sec) -> int {
+        int st = set_tuning_locked(b, &t);
+        if (st != SPM_OK) return st;
+        double bestt = 1e300;
+        for (int r = 0; r < reps + 1; ++r) {  // first pass warms up (graph capture, caches)
+            const double t0 = now_seconds();
+            st = drive(b, cnt, rq, nullptr, out.data(), nullptr, nullptr);
+            const double dt = now_seconds() - t0;
+            if (st != SPM_OK) return st;
+            if (r > 0) bestt = std::min(bestt, dt);
+        }
+        *sec = bestt;
+        return SPM_OK;
+    };
+    spm_batch_tuning cur = b->tune, win = cur;
+    double win_s = 1e300;
+    int st = time_it(cur, count, sample, &win_s);
+    if (st != SPM_OK) return st;
+    // 1) kernel organisation and persistent grid size
+    for (int fused = 0; fused <= 1; ++fused) {
+        for (int wpc : {0, 4, 8, 12, 16}) {
+            spm_batch_tuning t = win;
+            t.fused = fused;
+            t.waves_per_cu = wpc;
+            double s = 0;
+            st = time_it(t, count, sample, &s);
+            if (st != SPM_OK) return st;
+            if (s < win_s) {
+                win_s = s;
+                win = t;
+            }
+        }
+    }
+    // 2) chunk size per kind, timed on that kind's share of the sample
+    for (int k = 1; k < 4; ++k) {
+        std::vector<spm_request> sub;
+        for (int i = 0; i < count; ++i)
+            if (sample[i].kind == k) sub.push_back(sample[i]);
+        if (sub.empty()) continue;
+        double base_s = 0;
+        st = time_it(win, (int)sub.size(), sub.data(), &base_s);
+        if (st != SPM_OK) return st;
+        int best_L = win.log2_chunk[k];
+        for (int L = 6; L <= 13; ++L) {
+            if (L == win.log2_chunk[k]) continue;
+            spm_batch_tuning t = win;
+            t.log2_chunk[k] = L;
+            double s = 0;
+            st = time_it(t, (int)sub.size(), sub.data(), &s);
+            if (st != SPM_OK) return st;
+            if (s < base_s) {
+                base_s = s;
+                best_L = L;
+            }
+        }
+        win.log2_chunk[k] = best_L;
+    }
+    st = time_it(win, count, sample, &win_s);
+    if (st != SPM_OK) return st;
+    if (best) *best = win;
+    if (best_seconds) *best_seconds = win_s;
+    return SPM_OK;
+}
+
+extern "C" int spm_serve_submit(spm_batch* b, const spm_request* req, uint64_t* ticket)
+{
+    if (!b) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    return submit(b, req, now_seconds(), -1, ticket);
+}
+
+extern "C" int spm_serve_progress(spm_batch* b, int* in_flight)
+{
+    if (!b) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    return progress(b, in_flight);
+}
+
+extern "C" int spm_serve_poll(spm_batch* b, spm_completion* out, int max, int* got)
+{
+    if (!b || !got || max < 0 || (max > 0 && !out)) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    progress(b, nullptr);
+    int k = 0;
+    while (k < max && !b->done.empty()) {
+        out[k++] = b->done.front();
+        b->done.pop_front();
+    }
+    *got = k;
+    return SPM_OK;
+}
+
+extern "C" int spm_serve_cancel(spm_batch* b, uint64_t ticket)
+{
+    if (!b) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    auto it = b->live.find(ticket);
+    if (it == b->live.end()) return SPM_EINVAL;
+    it->second->cancelled = true;
+    return SPM_OK;
+}
+
+extern "C" int spm_serve_pending(spm_batch* b, int* queued, int* in_flight)
+{
+    if (!b) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (queued) *queued = (int)b->pending.size();
+    if (in_flight) {
+        int k = 0;
+        for (const Slot& s : b->slots) k += s.busy ? (int)s.reqs.size() : 0;
+        *in_flight = k;
+    }
+    return SPM_OK;
+}
+
+extern "C" int spm_serve_replay(spm_batch* b, int count, const spm_request* reqs, const double* arrival_s,
+                                spm_result* out, double* latency_s, spm_serve_stats* stats)
+{
+    if (!b || !arrival_s) return SPM_EINVAL;
+    for (int i = 1; i < count; ++i)
+        if (arrival_s[i] < arrival_s[i - 1]) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    return drive(b, count, reqs, arrival_s, out, latency_s, stats);
+}
+
+extern "C" int spm_batch_inject_fault(spm_batch* b, int launches)
+{
+    if (!b || launches < 0) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    b->fault_launches = launches;
+    return SPM_OK;
+}

@@ -227,25 +227,11 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
         std::snprintf(kn, sizeof kn, "dense_c128_kernel<%d, %s>", n, comp ? "true" : "false");
     p->kname = kn;
 
-    // Host preparation (dpSingleGPU.cuh:62-82; gpuComputeComplex.cu:212-241):
-    // x(empty) = a[i][n-1] - rowsum_i / 2 with the row sum taken left to right
-    // (re and im separately), and the column-major matrix with a negated copy
-    // (the kernel picks the plane by flip direction).
+    // Host preparation (dpSingleGPU.cuh:62-82; gpuComputeComplex.cu:212-241).
     const size_t col = (size_t)w * n;   // doubles per column
     const size_t plane = col * n;
     std::vector<double> colmaj(2 * plane), x0(col);
-    for (int i = 0; i < n; ++i) {
-        for (int c = 0; c < w; ++c) {
-            double rowSum = 0;
-            for (int j = 0; j < n; ++j) rowSum += a[(size_t)w * (i * n + j) + c];
-            x0[(size_t)w * i + c] = a[(size_t)w * (i * n + n - 1) + c] - (rowSum / 2);
-            for (int j = 0; j < n; ++j) {
-                const double v = a[(size_t)w * (i * n + j) + c];
-                colmaj[j * col + (size_t)w * i + c] = v;
-                colmaj[plane + j * col + (size_t)w * i + c] = -v;
-            }
-        }
-    }
+    dense_prepare(a, n, w, colmaj.data(), x0.data());
     SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * plane));
     SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * col));
     SPM_TRY(hipMalloc(&p->d_partial, sizeof(double) * p->rec * p->grid));
@@ -259,33 +245,12 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     return SPM_OK;
 }
 
-// Integer entry checks shared by the exact kinds: |a| <= 2^30 and absolute
-// row sums < 2^30 (so |y| fits int32), y0 = y(empty set), and log2 of the
-// bound on |T| = 2^(n-1) |perm|: Bregman-Minc for 0/1 matrices, the product
-// of absolute row sums otherwise.  The mod-2^128 sum is T itself only when
-// |T| < 2^127 (checked with margin for the log arithmetic).
+// Integer checks and y(empty set) (prep.cpp), into a vector.
 static int int_prepare(const int64_t* a, int n, std::vector<int>& y0, double* log2_bound)
 {
     if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
     y0.assign(n, 0);
-    bool binary = true;
-    for (int e = 0; e < n * n; ++e) binary = binary && (a[e] == 0 || a[e] == 1);
-    double lb = n - 1;
-    for (int i = 0; i < n; ++i) {
-        int64_t rs = 0, ra = 0;
-        for (int j = 0; j < n; ++j) {
-            const int64_t v = a[i * n + j];
-            if (v > (int64_t(1) << 30) || v < -(int64_t(1) << 30)) return SPM_ERANGE;
-            rs += v;
-            ra += v < 0 ? -v : v;
-        }
-        if (ra >= (int64_t(1) << 30)) return SPM_ERANGE;
-        y0[i] = (int)(2 * a[i * n + n - 1] - rs);
-        if (ra == 0) lb = -1e300;  // a zero row: perm = 0
-        else lb += binary ? std::lgamma((double)ra + 1.0) / ((double)ra * std::log(2.0)) : std::log2((double)ra);
-    }
-    *log2_bound = lb;
-    return lb >= 126.0 ? SPM_ERANGE : SPM_OK;
+    return spm::int_prepare(a, n, y0.data(), log2_bound);
 }
 
 // Exact integer plans (kind 3): y = 2x in int32, T mod 2^128.
@@ -333,11 +298,7 @@ static int make_int_plan(const int64_t* a, int n, int device, spm_plan** out)
     p->kname = kn;
     const size_t nn = (size_t)n * n;
     std::vector<int> colmaj(2 * nn);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-            colmaj[(size_t)j * n + i] = (int)(2 * a[i * n + j]);
-            colmaj[nn + (size_t)j * n + i] = (int)(-2 * a[i * n + j]);
-        }
+    int_planes(a, n, colmaj.data());
     SPM_TRY(hipMalloc(&p->d_M, sizeof(int) * 2 * nn));
     SPM_TRY(hipMalloc(&p->d_x0, sizeof(int) * n));
     SPM_TRY(hipMalloc(&p->d_partial, sizeof(unsigned long long) * 2 * p->grid));
@@ -515,46 +476,13 @@ static int dense_run(spm_plan* p, hipStream_t st, spm_result* r)
 
     std::memset(r, 0, sizeof *r);
     const int n = p->n;
-    const double f = (double)(4 * (n % 2) - 2);  // 2 (-1)^(n-1), dpSingleGPU.cuh:218
-    const double hi = p->h_out[0], lo = p->h_out[1];
-    r->hi_re = f * hi;
-    r->lo_re = f * lo;
-    r->re = f * (hi + lo);
-    r->abs_sum = 2.0 * p->h_out[2];
-    if (p->comps == 2) {
-        r->hi_im = f * p->h_out[3];
-        r->lo_im = f * p->h_out[4];
-        r->im = f * (p->h_out[3] + p->h_out[4]);
-    }
-    if (p->precision != SPM_PREC_COMPENSATED) r->err_est = -1.0;
-    else r->err_est = p->kind == 1 ? dense_err_est(n, r->abs_sum) : complex_err_est(n, r->abs_sum);
+    dense_fill_result(n, p->comps == 2, p->precision == SPM_PREC_COMPENSATED, p->h_out, r);
     r->status = SPM_OK;
     r->kernel_ms = ms;
     r->subsets = uint64_t(1) << (n - 1);
     r->n = n;
     r->launches = launches;
     r->seconds = now_seconds() - t0;
-    return SPM_OK;
-}
-
-// Fill `r` from T mod 2^128 of an integer run.  T must be a multiple of
-// 2^(n-1) (it is 2^(n-1) (-1)^(n-1) perm); a violation means a corrupted
-// run and is reported as a HIP error.
-static int int_result(const spm_plan* p, unsigned long long lo, unsigned long long hi, spm_result* r)
-{
-    const int n = p->n;
-    const unsigned __int128 T = ((unsigned __int128)hi << 64) | lo;
-    const unsigned __int128 mask = n >= 2 ? (((unsigned __int128)1 << (n - 1)) - 1) : 0;
-    if (T & mask) return SPM_EHIP;
-    __int128 perm = ((__int128)T) >> (n - 1);
-    if ((n - 1) & 1) perm = -perm;
-    r->exact_lo = (int64_t)(uint64_t)(unsigned __int128)perm;
-    r->exact_hi = (int64_t)(uint64_t)(((unsigned __int128)perm) >> 64);
-    r->is_exact = 1;
-    r->re = (double)perm;
-    r->hi_re = r->re;
-    r->lo_re = (double)(perm - (__int128)r->re);
-    r->err_est = 0.0;
     return SPM_OK;
 }
 
@@ -595,7 +523,7 @@ static int int_run(spm_plan* p, hipStream_t st, spm_result* r)
     std::memset(r, 0, sizeof *r);
     unsigned long long w[2];
     std::memcpy(w, p->h_out, 16);
-    const int rc = int_result(p, w[0], w[1], r);
+    const int rc = int_fill_result(p->n, w[0], w[1], r);
     if (rc != SPM_OK) return rc;
     r->status = SPM_OK;
     r->kernel_ms = ms;

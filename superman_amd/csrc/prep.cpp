@@ -1,0 +1,117 @@
+// prep.cpp -- host preparation of one matrix for the Gray-walk kernels and
+// the conversion of their reduced sums into results; shared by the
+// single-matrix plans (capi.cpp) and the batch engine (batch.cpp).
+#include "superman_amd.h"
+
+#include "spm_host.h"
+#include "spm_kernels.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace spm {
+
+// dpSingleGPU.cuh:62-82 / gpuComputeComplex.cu:212-241: x(empty) =
+// a[i][n-1] - rowsum_i / 2 with the row sum taken left to right (re and im
+// separately), and the column-major matrix with a negated copy (the kernels
+// pick the plane by flip direction).  `w` = doubles per entry (1 real, 2
+// complex interleaved); planes = 2 w n^2 doubles, x0 = w n doubles.
+void dense_prepare(const double* a, int n, int w, double* planes, double* x0)
+{
+    const size_t col = (size_t)w * n;
+    const size_t plane = col * n;
+    for (int i = 0; i < n; ++i) {
+        for (int c = 0; c < w; ++c) {
+            double rowSum = 0;
+            for (int j = 0; j < n; ++j) rowSum += a[(size_t)w * (i * n + j) + c];
+            x0[(size_t)w * i + c] = a[(size_t)w * (i * n + n - 1) + c] - (rowSum / 2);
+            for (int j = 0; j < n; ++j) {
+                const double v = a[(size_t)w * (i * n + j) + c];
+                planes[j * col + (size_t)w * i + c] = v;
+                planes[plane + j * col + (size_t)w * i + c] = -v;
+            }
+        }
+    }
+}
+
+// Integer entry checks: |a| <= 2^30 and absolute row sums < 2^30 (so |y|
+// fits int32), y0 = y(empty set), and log2 of the bound on
+// |T| = 2^(n-1) |perm|: Bregman-Minc for 0/1 matrices, the product of
+// absolute row sums otherwise.  The mod-2^128 sum is T itself only when
+// |T| < 2^127 (checked with margin for the log arithmetic).
+int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound)
+{
+    if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
+    bool binary = true;
+    for (int e = 0; e < n * n; ++e) binary = binary && (a[e] == 0 || a[e] == 1);
+    double lb = n - 1;
+    for (int i = 0; i < n; ++i) {
+        int64_t rs = 0, ra = 0;
+        for (int j = 0; j < n; ++j) {
+            const int64_t v = a[i * n + j];
+            if (v > (int64_t(1) << 30) || v < -(int64_t(1) << 30)) return SPM_ERANGE;
+            rs += v;
+            ra += v < 0 ? -v : v;
+        }
+        if (ra >= (int64_t(1) << 30)) return SPM_ERANGE;
+        y0[i] = (int)(2 * a[i * n + n - 1] - rs);
+        if (ra == 0) lb = -1e300;  // a zero row: perm = 0
+        else lb += binary ? std::lgamma((double)ra + 1.0) / ((double)ra * std::log(2.0)) : std::log2((double)ra);
+    }
+    *log2_bound = lb;
+    return lb >= 126.0 ? SPM_ERANGE : SPM_OK;
+}
+
+// +2A and -2A, column-major int32 (2 n^2 values).
+void int_planes(const int64_t* a, int n, int* planes)
+{
+    const size_t nn = (size_t)n * n;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            planes[(size_t)j * n + i] = (int)(2 * a[i * n + j]);
+            planes[nn + (size_t)j * n + i] = (int)(-2 * a[i * n + j]);
+        }
+}
+
+// rec = (hi, lo, abs) for real, (re hi, re lo, abs, im hi, im lo) for
+// complex: the compensated sum over all 2^(n-1) terms.
+void dense_fill_result(int n, bool complex_kind, bool compensated, const double* rec, spm_result* r)
+{
+    const double f = (double)(4 * (n % 2) - 2);  // 2 (-1)^(n-1), dpSingleGPU.cuh:218
+    r->hi_re = f * rec[0];
+    r->lo_re = f * rec[1];
+    r->re = f * (rec[0] + rec[1]);
+    r->abs_sum = 2.0 * rec[2];
+    if (complex_kind) {
+        r->hi_im = f * rec[3];
+        r->lo_im = f * rec[4];
+        r->im = f * (rec[3] + rec[4]);
+    }
+    if (!compensated) r->err_est = -1.0;
+    else r->err_est = complex_kind ? complex_err_est(n, r->abs_sum) : dense_err_est(n, r->abs_sum);
+    r->subsets = uint64_t(1) << (n - 1);
+    r->n = n;
+}
+
+// T mod 2^128 -> perm = (-1)^(n-1) T / 2^(n-1).  T must be a multiple of
+// 2^(n-1); a violation means a corrupted run and is reported as a HIP error.
+int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r)
+{
+    const unsigned __int128 T = ((unsigned __int128)hi << 64) | lo;
+    const unsigned __int128 mask = n >= 2 ? (((unsigned __int128)1 << (n - 1)) - 1) : 0;
+    if (T & mask) return SPM_EHIP;
+    __int128 perm = ((__int128)T) >> (n - 1);
+    if ((n - 1) & 1) perm = -perm;
+    r->exact_lo = (int64_t)(uint64_t)(unsigned __int128)perm;
+    r->exact_hi = (int64_t)(uint64_t)(((unsigned __int128)perm) >> 64);
+    r->is_exact = 1;
+    r->re = (double)perm;
+    r->hi_re = r->re;
+    r->lo_re = (double)(perm - (__int128)r->re);
+    r->err_est = 0.0;
+    r->subsets = uint64_t(1) << (n - 1);
+    r->n = n;
+    return SPM_OK;
+}
+
+}  // namespace spm

@@ -1,0 +1,50 @@
+// spm_batch.h -- device-visible layout of a batch arena, shared by the
+// persistent batch kernels (batch.hip) and the host engine (batch_engine.cpp).
+#pragma once
+
+#include <stdint.h>
+
+namespace spm {
+
+constexpr int kBatchMaxN = 28;      // orders handled by the batched kernel
+constexpr int kBatchReal = 1;       // compensated FP64
+constexpr int kBatchComplex = 2;    // compensated complex128
+constexpr int kBatchInt = 3;        // exact integer (mod 2^128)
+constexpr int kBatchRec = 8;        // doubles per item / result record
+constexpr int kBatchAlign = 64;     // byte alignment of every arena region
+
+struct BatchHeader {                // 64 bytes at arena offset 0
+    uint32_t requests;
+    uint32_t items;
+    uint64_t desc_off;              // BatchRequestDesc[requests]
+    uint64_t item_off;              // uint32_t[items]: item -> request
+    uint32_t range_begin[4];        // item range walked by kernel q: q = 0 all
+    uint32_t range_end[4];          // kinds (fused kernel), q = kind (one-kind kernels)
+    uint64_t pad;
+};
+
+struct BatchRequestDesc {           // 64 bytes
+    uint32_t kind;
+    uint32_t n;
+    int32_t L;                      // lane chunk = 2^L Gray steps
+    uint32_t items;                 // ceil(num_chunks / 64)
+    uint64_t num_chunks;            // 2^(n-1-L)
+    uint64_t mat_off;               // +A/-A planes (f64, c128) or +2A/-2A (int32)
+    uint64_t x0_off;                // x(empty set) / y(empty set)
+    uint32_t item_base;             // first item of this request
+    uint32_t pad[5];
+};
+
+static_assert(sizeof(BatchHeader) == 64, "header layout");
+static_assert(sizeof(BatchRequestDesc) == 64, "descriptor layout");
+
+// batch.hip
+//   walk:     (const uint8_t* arena, double* item_rec, unsigned int* counter);
+//             `kinds` = 7 (every kind in one kernel) or 1, 2, 4 (one kind)
+//   finalize: (const uint8_t* arena, const double* item_rec, double* result_rec)
+//   reset:    (unsigned int* counter)
+const void* batch_walk_kernel_ptr(int kinds);
+const void* batch_finalize_kernel_ptr();
+const void* batch_reset_kernel_ptr();
+
+}  // namespace spm

@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "superman_amd.h"
+
 namespace spm {
 
 constexpr int kWaveLanes = 64;
@@ -37,6 +39,13 @@ inline double complex_err_est(int n, double abs_sum)
     const double u = 1.1102230246251565e-16;
     return 4.0 * n * u * abs_sum;
 }
+
+// prep.cpp -- per-matrix host preparation and result conversion
+void dense_prepare(const double* a, int n, int w, double* planes, double* x0);
+int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound);
+void int_planes(const int64_t* a, int n, int* planes);
+void dense_fill_result(int n, bool complex_kind, bool compensated, const double* rec, spm_result* r);
+int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r);
 
 // codegen.cpp -- SpaRyser per-pattern kernels
 void ut_order_mask(const unsigned char* nz, int n, std::vector<int>& rowPerm, std::vector<int>& colPerm);
