@@ -150,11 +150,13 @@ batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_r
     const uint32_t* item_req = (const uint32_t*)(arena + hd->item_off);
     const uint32_t lane = threadIdx.x & (kWave - 1);
 
-    for (;;) {
-        uint32_t i = 0;
-        if (lane == 0) i = begin + atomicAdd(counter + q, 1u);
-        i = uniform(i);
-        if (i >= end) break;
+    // The loop body has no lane-divergent control flow at its own level: the
+    // work counter is bumped by every lane (lane 0 adds 1, the others 0) and
+    // every lane stores the (identical) wave total.  A lane-0-only branch
+    // here lets the compiler's control-flow structurizer split the loop so
+    // that the item index is read from a partial wave.
+    uint32_t i = begin + uniform(atomicAdd(counter + q, lane == 0 ? 1u : 0u));
+    while (i < end) {
         const BatchRequestDesc* d = reqs + uniform(item_req[i]);
         Item it;
         it.M = arena + uniform(d->mat_off);
@@ -169,16 +171,14 @@ batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_r
         if constexpr ((KINDS & 1) != 0) if (kind == kBatchReal) dispatch<kBatchReal>(n, it, o, orders);
         if constexpr ((KINDS & 2) != 0) if (kind == kBatchComplex) dispatch<kBatchComplex>(n, it, o, orders);
         if constexpr ((KINDS & 4) != 0) if (kind == kBatchInt) dispatch<kBatchInt>(n, it, o, orders);
-        if (it.h >= num_chunks) {  // lanes past the matrix's last chunk
+        const bool idle = it.h >= num_chunks;  // lanes past the matrix's last chunk
 #pragma unroll
-            for (int q = 0; q < 8; ++q) o[q] = 0.0;
-        }
+        for (int c = 0; c < 8; ++c) o[c] = idle ? 0.0 : o[c];
         wave_combine(kind, o);
-        if (lane == 0) {
-            double* rec = item_rec + (size_t)i * kBatchRec;
+        double* rec = item_rec + (size_t)i * kBatchRec;
 #pragma unroll
-            for (int q = 0; q < kBatchRec; ++q) rec[q] = o[q];
-        }
+        for (int c = 0; c < kBatchRec; ++c) rec[c] = o[c];
+        i = begin + uniform(atomicAdd(counter + q, lane == 0 ? 1u : 0u));
     }
 }
 
