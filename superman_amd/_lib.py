@@ -30,6 +30,10 @@ SPM_CG_REGISTER_EFFICIENT = 1
 SPM_PREC_PLAIN = 0
 SPM_PREC_COMPENSATED = 1
 
+SPM_INT_WALK_AUTO = 0
+SPM_INT_WALK_INT32 = 1
+SPM_INT_WALK_FP64 = 2
+
 SPM_KIND_REAL = 1
 SPM_KIND_COMPLEX = 2
 SPM_KIND_INT = 3
@@ -109,6 +113,7 @@ PROTOTYPES = {
     "spm_dense_c128": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_int_exact": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_plan_int": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "spm_plan_int_walk": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
     "spm_sparse_codegen": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_codegen_source": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
                                           ctypes.POINTER(ctypes.c_size_t)]),

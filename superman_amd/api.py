@@ -137,13 +137,16 @@ class DensePlan:
     """A permanent with the matrix resident on the device: FP64, complex128
     for a complex matrix, or exact integer with precision="exact"."""
 
-    def __init__(self, a, precision="kahan", device: int = 0):
+    INT_WALKS = {"auto": _lib.SPM_INT_WALK_AUTO, "int32": _lib.SPM_INT_WALK_INT32, "fp64": _lib.SPM_INT_WALK_FP64}
+
+    def __init__(self, a, precision="kahan", device: int = 0, int_walk: str = "auto"):
         self.complex = np.iscomplexobj(a)
         self._h = ctypes.c_void_p()
         if precision == "exact":
             a = _square(a, np.int64)
-            check(lib().spm_plan_int(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.shape[0], device,
-                                     ctypes.byref(self._h)), "spm_plan_int")
+            check(lib().spm_plan_int_walk(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.shape[0],
+                                          self.INT_WALKS[int_walk], device, ctypes.byref(self._h)),
+                  "spm_plan_int_walk")
         elif self.complex:
             a = _square(a, np.complex128)
             check(lib().spm_plan_dense_c128(a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),

@@ -36,6 +36,7 @@ SOURCES = [
     ("dense_f64.hip", []),
     ("dense_c128.hip", []),
     ("int_dense.hip", []),
+    ("int_small.hip", []),
     ("batch.hip", []),
     ("capi.cpp", []),
     ("prep.cpp", []),

@@ -11,20 +11,24 @@
 // from a device counter (one atomic per item), dispatches on the item's kind
 // and order (wave-uniform) and writes the item's wave-reduced sum to its own
 // record.  A wave leaves when the counter passes the item count, so every
-// wave reaches the exit.  A second kernel reduces each matrix's item records
-// in a fixed order: the result of a matrix depends only on its data and its
-// chunk size 2^L, never on the batch it rides in or on scheduling.
+// wave reaches the exit.  The wave that completes a matrix's last item (a
+// per-matrix count) reduces that matrix's item records in a fixed order:
+// the result of a matrix depends only on its data and its chunk size 2^L,
+// never on the batch it rides in or on scheduling.
 //
 // Arena layout (byte offsets in the header, all 64-byte aligned):
 //   header | request descriptors | item -> request table | matrix data
 // Records: item_rec[item][8], result_rec[request][8] (doubles):
 //   real    : hi, lo, abs
 //   complex : re hi, re lo, abs, im hi, im lo
-//   integer : T mod 2^128 as (lo, hi) 64-bit words
+//   integer : T mod 2^128 as (lo, hi) 64-bit words (both integer walks:
+//             int32 for general entries, FP64 when every absolute row sum
+//             is below 2^(bit width of n), e.g. 0/1 matrices)
 #include "spm_batch.h"
 #include "walk_c128.h"
 #include "walk_f64.h"
 #include "walk_int.h"
+#include "walk_intfp.h"
 
 #include <utility>
 
@@ -86,12 +90,21 @@ __device__ __forceinline__ void item_int(const Item& it, double (&o)[8])
     o[1] = __longlong_as_double((long long)(unsigned long long)(acc >> 64));
 }
 
+template <int N>
+__device__ __forceinline__ void item_int_small(const Item& it, double (&o)[8])
+{
+    const unsigned __int128 acc = real::walk_intfp<N>((const double*)it.M, (const double*)it.x0, it.h, it.L);
+    o[0] = __longlong_as_double((long long)(unsigned long long)acc);
+    o[1] = __longlong_as_double((long long)(unsigned long long)(acc >> 64));
+}
+
 template <int KIND, int N>
 __device__ __forceinline__ void run_item(const Item& it, double (&o)[8])
 {
     if constexpr (KIND == kBatchReal) item_real<N>(it, o);
     else if constexpr (KIND == kBatchComplex) item_complex<N>(it, o);
-    else item_int<N>(it, o);
+    else if constexpr (KIND == kBatchInt) item_int<N>(it, o);
+    else item_int_small<N>(it, o);
 }
 
 // Order dispatch: a chain of uniform compares, one inlined walk per order.
@@ -117,7 +130,7 @@ __device__ __forceinline__ void reduce_i128_wave(unsigned long long& lo, unsigne
 // butterfly order; every lane ends with the total.
 __device__ __forceinline__ void wave_combine(int kind, double (&o)[8])
 {
-    if (kind == kBatchInt) {
+    if (kind == kBatchInt || kind == kBatchIntSmall) {
         unsigned long long lo = (unsigned long long)__double_as_longlong(o[0]);
         unsigned long long hi = (unsigned long long)__double_as_longlong(o[1]);
         reduce_i128_wave(lo, hi);
@@ -132,18 +145,48 @@ __device__ __forceinline__ void wave_combine(int kind, double (&o)[8])
     }
 }
 
+// A request's item records summed in a fixed order (lane l takes items
+// l, l+64, ..., then the butterfly); every lane writes the same total.
+__device__ __forceinline__ void reduce_request(int kind, const double* __restrict__ recs, uint32_t items,
+                                               uint32_t lane, double* __restrict__ out)
+{
+    double o[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    unsigned __int128 t = 0;
+    const bool is_int = kind == kBatchInt || kind == kBatchIntSmall;
+    for (uint32_t j = lane; j < items; j += kWave) {
+        const double* rec = recs + (size_t)j * kBatchRec;
+        if (is_int) {
+            t += ((unsigned __int128)(unsigned long long)__double_as_longlong(rec[1]) << 64) |
+                 (unsigned long long)__double_as_longlong(rec[0]);
+        } else {
+            dd_add(o[0], o[1], rec[0], rec[1]);
+            o[2] += rec[2];
+            if (kind == kBatchComplex) dd_add(o[3], o[4], rec[3], rec[4]);
+        }
+    }
+    if (is_int) {
+        o[0] = __longlong_as_double((long long)(unsigned long long)t);
+        o[1] = __longlong_as_double((long long)(unsigned long long)(t >> 64));
+    }
+    wave_combine(kind, o);
+#pragma unroll
+    for (int c = 0; c < kBatchRec; ++c) out[c] = o[c];
+}
+
 }  // namespace batch
 
 // KINDS: bit mask of the kinds this instantiation walks (1 real, 2 complex,
-// 4 integer).  The fused kernel (7) walks header range 0 with counter 0; a
-// one-kind kernel walks its kind's range with that kind's counter.
+// 4 integer int32 walk, 8 integer FP64 walk).  The fused kernel (15) walks
+// header range 0 with counter 0; the real (1), complex (2) and integer (12)
+// kernels walk range and counter 1, 2 and 3.
 template <int KINDS>
 __global__ void __launch_bounds__(kDenseBlock)
-batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_rec, unsigned int* __restrict__ counter)
+batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_rec, unsigned int* __restrict__ counter,
+                  unsigned int* __restrict__ done, double* __restrict__ result_rec)
 {
     using namespace batch;
     const BatchHeader* hd = (const BatchHeader*)arena;
-    constexpr int q = KINDS == 7 ? 0 : KINDS == 1 ? kBatchReal : KINDS == 2 ? kBatchComplex : kBatchInt;
+    constexpr int q = KINDS == 15 ? 0 : KINDS == 1 ? 1 : KINDS == 2 ? 2 : 3;
     const uint32_t begin = uniform(hd->range_begin[q]);
     const uint32_t end = uniform(hd->range_end[q]);
     const BatchRequestDesc* reqs = (const BatchRequestDesc*)(arena + hd->desc_off);
@@ -171,6 +214,7 @@ batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_r
         if constexpr ((KINDS & 1) != 0) if (kind == kBatchReal) dispatch<kBatchReal>(n, it, o, orders);
         if constexpr ((KINDS & 2) != 0) if (kind == kBatchComplex) dispatch<kBatchComplex>(n, it, o, orders);
         if constexpr ((KINDS & 4) != 0) if (kind == kBatchInt) dispatch<kBatchInt>(n, it, o, orders);
+        if constexpr ((KINDS & 8) != 0) if (kind == kBatchIntSmall) dispatch<kBatchIntSmall>(n, it, o, orders);
         const bool idle = it.h >= num_chunks;  // lanes past the matrix's last chunk
 #pragma unroll
         for (int c = 0; c < 8; ++c) o[c] = idle ? 0.0 : o[c];
@@ -178,66 +222,40 @@ batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_r
         double* rec = item_rec + (size_t)i * kBatchRec;
 #pragma unroll
         for (int c = 0; c < kBatchRec; ++c) rec[c] = o[c];
+        // The wave that completes a request's last item reduces its records
+        // (release/acquire through the per-request count).
+        __threadfence();
+        const uint32_t r = uniform(item_req[i]);
+        const uint32_t items = uniform(d->items);
+        const uint32_t prev = uniform(atomicAdd(done + r, lane == 0 ? 1u : 0u));
+        if (prev + 1 == items) {
+            __threadfence();
+            reduce_request(kind, item_rec + (size_t)uniform(d->item_base) * kBatchRec, items, lane,
+                           result_rec + (size_t)r * kBatchRec);
+        }
         i = begin + uniform(atomicAdd(counter + q, lane == 0 ? 1u : 0u));
     }
 }
 
-// One wave per request: the request's item records in a fixed order (lane l
-// takes items l, l+64, ..., then the butterfly).
-__global__ void __launch_bounds__(kDenseBlock)
-batch_finalize_kernel(const uint8_t* __restrict__ arena, const double* __restrict__ item_rec,
-                      double* __restrict__ result_rec)
+// Resets the four work counters and the per-request item counts (a kernel,
+// so that a captured graph holds no memset node).
+__global__ void batch_reset_kernel(unsigned int* counter, unsigned int* done, unsigned int n_done)
 {
-    using namespace batch;
-    const BatchHeader* hd = (const BatchHeader*)arena;
-    const uint32_t r = blockIdx.x * (kDenseBlock / kWave) + threadIdx.x / kWave;
-    if (r >= hd->requests) return;
-    const BatchRequestDesc* d = (const BatchRequestDesc*)(arena + hd->desc_off) + r;
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const int kind = (int)d->kind;
-    double o[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    unsigned __int128 t = 0;
-    for (uint32_t j = lane; j < d->items; j += kWave) {
-        const double* rec = item_rec + (size_t)(d->item_base + j) * kBatchRec;
-        if (kind == kBatchInt) {
-            t += ((unsigned __int128)(unsigned long long)__double_as_longlong(rec[1]) << 64) |
-                 (unsigned long long)__double_as_longlong(rec[0]);
-        } else {
-            dd_add(o[0], o[1], rec[0], rec[1]);
-            o[2] += rec[2];
-            if (kind == kBatchComplex) dd_add(o[3], o[4], rec[3], rec[4]);
-        }
-    }
-    if (kind == kBatchInt) {
-        o[0] = __longlong_as_double((long long)(unsigned long long)t);
-        o[1] = __longlong_as_double((long long)(unsigned long long)(t >> 64));
-    }
-    wave_combine(kind, o);
-    if (lane == 0) {
-        double* out = result_rec + (size_t)r * kBatchRec;
-#pragma unroll
-        for (int q = 0; q < kBatchRec; ++q) out[q] = o[q];
-    }
-}
-
-// Resets the four work counters (a kernel, so that a captured graph holds no
-// memset node).
-__global__ void batch_reset_kernel(unsigned int* counter)
-{
-    if (threadIdx.x < 4) counter[threadIdx.x] = 0u;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 4) counter[t] = 0u;
+    for (uint32_t j = t; j < n_done; j += gridDim.x * blockDim.x) done[j] = 0u;
 }
 
 const void* batch_walk_kernel_ptr(int kinds)
 {
     switch (kinds) {
-    case 7: return reinterpret_cast<const void*>(&batch_walk_kernel<7>);
+    case 15: return reinterpret_cast<const void*>(&batch_walk_kernel<15>);
     case 1: return reinterpret_cast<const void*>(&batch_walk_kernel<1>);
     case 2: return reinterpret_cast<const void*>(&batch_walk_kernel<2>);
-    case 4: return reinterpret_cast<const void*>(&batch_walk_kernel<4>);
+    case 12: return reinterpret_cast<const void*>(&batch_walk_kernel<12>);
     default: return nullptr;
     }
 }
-const void* batch_finalize_kernel_ptr() { return reinterpret_cast<const void*>(&batch_finalize_kernel); }
 const void* batch_reset_kernel_ptr() { return reinterpret_cast<const void*>(&batch_reset_kernel); }
 
 }  // namespace spm

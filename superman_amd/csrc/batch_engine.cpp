@@ -5,7 +5,7 @@
 // pinned staging buffer, a device arena (header, request descriptors, item
 // table, matrix data), item and result records, the work counters and,
 // optionally, a captured hipGraph of the batch's launch sequence
-//   reset counters -> walk kernel(s) -> finalize -> results to pinned host.
+//   reset counters -> walk kernel(s) -> results to pinned host.
 // Requests are prepared on the host at submission (x(empty), +/- planes;
 // integer checks), queue in FIFO order, and are admitted into the next batch
 // while it stays within the slot's request, item, byte and work budgets.
@@ -36,15 +36,25 @@ namespace {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// FP64-op units per Gray subset (work estimates for budgets and ordering).
-double ops_per_subset(int kind, int n)
+// FP64-op units per Gray subset (work estimates for budgets and ordering),
+// by device walk.
+double ops_per_subset(int dev_kind, int n)
 {
-    return kind == kBatchComplex ? 8.0 * n : kind == kBatchInt ? 3.0 * n : 2.0 * n;
+    switch (dev_kind) {
+    case kBatchComplex: return 8.0 * n;
+    case kBatchInt: return 12.0 * n;
+    case kBatchIntSmall: return 5.0 * n;
+    default: return 2.0 * n;
+    }
 }
+
+// Kernel range / counter of a device walk: 1 real, 2 complex, 3 integer.
+int walk_group(int dev_kind) { return dev_kind == kBatchIntSmall ? kBatchInt : dev_kind; }
 
 struct Req {
     uint64_t ticket = 0, tag = 0;
     int kind = 0, n = 0;
+    int dev_kind = 0;           // device walk (kBatch*)
     std::vector<uint8_t> data;  // planes, then x0 at x0_rel (64-byte aligned)
     size_t x0_rel = 0;
     int L = 0;
@@ -66,7 +76,7 @@ struct Slot {
     double* d_item = nullptr;
     double* d_res = nullptr;
     double* h_res = nullptr;
-    unsigned int* d_counter = nullptr;
+    unsigned int* d_counter = nullptr;  // [4] work counters, then [max_requests] item counts
     std::vector<Req*> reqs;
     bool busy = false;
     bool fault = false;
@@ -112,7 +122,7 @@ void geometry(const spm_batch* b, Req* r)
     r->L = L;
     r->chunks = uint64_t(1) << (B - L);
     r->items = (uint32_t)((r->chunks + kWaveLanes - 1) / kWaveLanes);
-    const double ops = ops_per_subset(r->kind, r->n);
+    const double ops = ops_per_subset(r->dev_kind, r->n);
     r->item_cost = std::ldexp(1.0, L) * kWaveLanes * ops + double(r->n) * r->n;
     r->work = std::ldexp(1.0, B) * ops;
 }
@@ -136,14 +146,28 @@ int prepare(const spm_request* q, Req* r)
             if (v != std::floor(v) || std::fabs(v) > 1073741824.0) return SPM_ENOTINT;
             ai[e] = (int64_t)v;
         }
-        const size_t planes = sizeof(int) * 2 * (size_t)n * n;
-        r->x0_rel = align_up(planes, kBatchAlign);
-        r->data.assign(r->x0_rel + sizeof(int) * n, 0);
+        std::vector<int> y0(n);
         double lb = 0;
-        const int st = int_prepare(ai.data(), n, (int*)(r->data.data() + r->x0_rel), &lb);
+        const int st = int_prepare(ai.data(), n, y0.data(), &lb);
         if (st != SPM_OK) return st;
-        int_planes(ai.data(), n, (int*)r->data.data());
+        if (int_small_ok(ai.data(), n)) {  // FP64 walk: planes and y0 as doubles
+            r->dev_kind = kBatchIntSmall;
+            const size_t planes = sizeof(double) * 2 * (size_t)n * n;
+            r->x0_rel = align_up(planes, kBatchAlign);
+            r->data.assign(r->x0_rel + sizeof(double) * n, 0);
+            int_small_planes(ai.data(), n, (double*)r->data.data());
+            double* yd = (double*)(r->data.data() + r->x0_rel);
+            for (int i = 0; i < n; ++i) yd[i] = (double)y0[i];
+        } else {
+            r->dev_kind = kBatchInt;
+            const size_t planes = sizeof(int) * 2 * (size_t)n * n;
+            r->x0_rel = align_up(planes, kBatchAlign);
+            r->data.assign(r->x0_rel + sizeof(int) * n, 0);
+            int_planes(ai.data(), n, (int*)r->data.data());
+            std::memcpy(r->data.data() + r->x0_rel, y0.data(), sizeof(int) * n);
+        }
     } else {
+        r->dev_kind = q->kind == SPM_KIND_COMPLEX ? kBatchComplex : kBatchReal;
         const size_t planes = sizeof(double) * 2 * (size_t)w * n * n;
         r->x0_rel = align_up(planes, kBatchAlign);
         r->data.assign(r->x0_rel + sizeof(double) * w * n, 0);
@@ -216,28 +240,25 @@ int grid_for(const spm_batch* b, int q)
 }
 
 // The launch sequence of one batch after its arena upload; `nreq` sizes the
-// finalize grid and the result copy (the graph uses the slot maximum).
+// result copy (the graph uses the slot maximum).  Each request's result is
+// reduced inside the walk kernel by the wave that finishes its last item.
 hipError_t enqueue(spm_batch* b, Slot& s, int nreq)
 {
     hipError_t e;
+    unsigned int* done = s.d_counter + 64;
     {
-        void* args[] = {&s.d_counter};
-        e = hipLaunchKernel(batch_reset_kernel_ptr(), dim3(1), dim3(kWaveLanes), args, 0, s.stream);
+        unsigned int n_done = (unsigned int)b->cfg.max_requests;
+        const int blocks = std::min(256, (b->cfg.max_requests + 255) / 256);
+        void* args[] = {&s.d_counter, &done, &n_done};
+        e = hipLaunchKernel(batch_reset_kernel_ptr(), dim3(blocks), dim3(256), args, 0, s.stream);
         if (e != hipSuccess) return e;
     }
-    const int kinds[3] = {1, 2, 4};
+    const int kinds[3] = {1, 2, 12};
     for (int k = 0; k < (b->tune.fused ? 1 : 3); ++k) {
-        const int mask = b->tune.fused ? 7 : kinds[k];
+        const int mask = b->tune.fused ? 15 : kinds[k];
         const int q = b->tune.fused ? 0 : k + 1;
-        void* args[] = {&s.d_arena, &s.d_item, &s.d_counter};
+        void* args[] = {&s.d_arena, &s.d_item, &s.d_counter, &done, &s.d_res};
         e = hipLaunchKernel(batch_walk_kernel_ptr(mask), dim3(grid_for(b, q)), dim3(kDenseBlock), args, 0, s.stream);
-        if (e != hipSuccess) return e;
-    }
-    {
-        const int waves = kDenseBlock / kWaveLanes;
-        void* args[] = {&s.d_arena, &s.d_item, &s.d_res};
-        e = hipLaunchKernel(batch_finalize_kernel_ptr(), dim3((nreq + waves - 1) / waves), dim3(kDenseBlock), args, 0,
-                            s.stream);
         if (e != hipSuccess) return e;
     }
     return hipMemcpyAsync(s.h_res, s.d_res, sizeof(double) * kBatchRec * nreq, hipMemcpyDeviceToHost, s.stream);
@@ -306,7 +327,8 @@ size_t stage(spm_batch* b, Slot& s, std::vector<Req*>& rs)
         std::stable_sort(rs.begin(), rs.end(), [](const Req* x, const Req* y) { return x->item_cost > y->item_cost; });
     } else {
         std::stable_sort(rs.begin(), rs.end(), [](const Req* x, const Req* y) {
-            return x->kind != y->kind ? x->kind < y->kind : x->item_cost > y->item_cost;
+            const int gx = walk_group(x->dev_kind), gy = walk_group(y->dev_kind);
+            return gx != gy ? gx < gy : x->item_cost > y->item_cost;
         });
     }
     uint8_t* h = s.h_stage;
@@ -322,10 +344,11 @@ size_t stage(spm_batch* b, Slot& s, std::vector<Req*>& rs)
     for (int k = 0; k < 4; ++k) hd.range_begin[k] = hd.range_end[k] = 0;
     for (size_t j = 0; j < rs.size(); ++j) {
         Req* r = rs[j];
-        if (hd.range_end[r->kind] == 0 && hd.range_begin[r->kind] == 0) hd.range_begin[r->kind] = item;
+        const int grp = walk_group(r->dev_kind);
+        if (hd.range_end[grp] == 0 && hd.range_begin[grp] == 0) hd.range_begin[grp] = item;
         BatchRequestDesc d;
         std::memset(&d, 0, sizeof d);
-        d.kind = (uint32_t)r->kind;
+        d.kind = (uint32_t)r->dev_kind;
         d.n = (uint32_t)r->n;
         d.L = r->L;
         d.items = r->items;
@@ -338,7 +361,7 @@ size_t stage(spm_batch* b, Slot& s, std::vector<Req*>& rs)
         data += align_up(r->data.size(), kBatchAlign);
         for (uint32_t t = 0; t < r->items; ++t) item_req[item + t] = (uint32_t)j;
         item += r->items;
-        hd.range_end[r->kind] = item;
+        hd.range_end[grp] = item;
     }
     hd.items = item;
     hd.range_begin[0] = 0;
@@ -617,7 +640,7 @@ extern "C" int spm_batch_create(const spm_batch_config* cfg, spm_batch** out)
     if (st != SPM_OK) return st;
     DeviceGuard g(cfg->device);
     if (!g.ok()) return SPM_EHIP;
-    const int masks[4] = {7, 1, 2, 4};
+    const int masks[4] = {15, 1, 2, 12};
     for (int q = 0; q < 4; ++q) {
         int bpc = 0;
         st = blocks_per_cu(cfg->device, batch_walk_kernel_ptr(masks[q]), kDenseBlock, &bpc);
@@ -637,7 +660,7 @@ extern "C" int spm_batch_create(const spm_batch_config* cfg, spm_batch** out)
              hipMalloc(&s.d_arena, b->stage_bytes) == hipSuccess &&
              hipMalloc(&s.d_item, sizeof(double) * kBatchRec * (size_t)cfg->max_items) == hipSuccess &&
              hipMalloc(&s.d_res, sizeof(double) * kBatchRec * cfg->max_requests) == hipSuccess &&
-             hipMalloc(&s.d_counter, 256) == hipSuccess;
+             hipMalloc(&s.d_counter, sizeof(unsigned int) * (64 + (size_t)cfg->max_requests)) == hipSuccess;
         if (!ok) {
             for (Slot& t : b->slots) free_slot(t);
             return SPM_ENOMEM;

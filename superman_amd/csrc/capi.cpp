@@ -254,14 +254,20 @@ static int int_prepare(const int64_t* a, int n, std::vector<int>& y0, double* lo
 }
 
 // Exact integer plans (kind 3): y = 2x in int32, T mod 2^128.
-static int make_int_plan(const int64_t* a, int n, int device, spm_plan** out)
+// walk: SPM_INT_WALK_AUTO picks the FP64 walk when every absolute row sum is
+// below 2^int_small_bits(n) (int_small_ok), else the int32 walk.
+static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan** out)
 {
     if (!out) return SPM_EINVAL;
     *out = nullptr;
+    if (walk != SPM_INT_WALK_AUTO && walk != SPM_INT_WALK_INT32 && walk != SPM_INT_WALK_FP64) return SPM_EINVAL;
     std::vector<int> y0;
     double lb = 0;
     int rc = int_prepare(a, n, y0, &lb);
     if (rc != SPM_OK) return rc;
+    const bool small = int_small_ok(a, n);
+    if (walk == SPM_INT_WALK_FP64 && !small) return SPM_ERANGE;
+    const bool fp = walk == SPM_INT_WALK_FP64 || (walk == SPM_INT_WALK_AUTO && small);
     int cus = 0;
     int st = device_cus(device, &cus);
     if (st != SPM_OK) return st;
@@ -274,7 +280,7 @@ static int make_int_plan(const int64_t* a, int n, int device, spm_plan** out)
     p->precision = -1;
     p->device = device;
     p->log2_bound = lb;
-    p->kern = int_dense_kernel_ptr(n);
+    p->kern = fp ? int_small_kernel_ptr(n) : int_dense_kernel_ptr(n);
     p->rec = 2;
     p->comps = 0;
     if (!p->kern) {
@@ -288,24 +294,33 @@ static int make_int_plan(const int64_t* a, int n, int device, spm_plan** out)
         return st;
     }
     const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
-    gray_geometry(n, n >= 3 ? 1 : 0, waves, &p->L, &p->chunks, &p->tasks);
+    gray_geometry(n, fp ? dense_f64_unroll_bits(n) : (n >= 3 ? 1 : 0), waves, &p->L, &p->chunks, &p->tasks);
     const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
     p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
-    const double est_s = std::ldexp(1.0, n - 1) * (2.0 * n) / 3.9e13;
+    const double est_s = std::ldexp(1.0, n - 1) * ((fp ? 5.0 : 12.0) * n) / 3.9e13;
     p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
     char kn[96];
-    std::snprintf(kn, sizeof kn, "int_dense_kernel<%d>", n);
+    std::snprintf(kn, sizeof kn, fp ? "int_small_kernel<%d>" : "int_dense_kernel<%d>", n);
     p->kname = kn;
     const size_t nn = (size_t)n * n;
-    std::vector<int> colmaj(2 * nn);
-    int_planes(a, n, colmaj.data());
-    SPM_TRY(hipMalloc(&p->d_M, sizeof(int) * 2 * nn));
-    SPM_TRY(hipMalloc(&p->d_x0, sizeof(int) * n));
+    if (fp) {
+        std::vector<double> colmaj(2 * nn), yd(y0.begin(), y0.end());
+        int_small_planes(a, n, colmaj.data());
+        SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * nn));
+        SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * n));
+        SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice));
+        SPM_TRY(hipMemcpy(p->d_x0, yd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    } else {
+        std::vector<int> colmaj(2 * nn);
+        int_planes(a, n, colmaj.data());
+        SPM_TRY(hipMalloc(&p->d_M, sizeof(int) * 2 * nn));
+        SPM_TRY(hipMalloc(&p->d_x0, sizeof(int) * n));
+        SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(int) * 2 * nn, hipMemcpyHostToDevice));
+        SPM_TRY(hipMemcpy(p->d_x0, y0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    }
     SPM_TRY(hipMalloc(&p->d_partial, sizeof(unsigned long long) * 2 * p->grid));
     SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 6));
     SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 6, hipHostMallocDefault));
-    SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(int) * 2 * nn, hipMemcpyHostToDevice));
-    SPM_TRY(hipMemcpy(p->d_x0, y0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     SPM_TRY(hipEventCreate(&p->ev0));
     SPM_TRY(hipEventCreate(&p->ev1));
     *out = p;
@@ -319,7 +334,12 @@ extern "C" int spm_plan_dense_f64(const double* a, int n, int precision, int dev
 
 extern "C" int spm_plan_int(const int64_t* a, int n, int device, spm_plan** out)
 {
-    return make_int_plan(a, n, device, out);
+    return make_int_plan(a, n, SPM_INT_WALK_AUTO, device, out);
+}
+
+extern "C" int spm_plan_int_walk(const int64_t* a, int n, int walk, int device, spm_plan** out)
+{
+    return make_int_plan(a, n, walk, device, out);
 }
 
 // Generated SpaRyser plans (kind 4).
@@ -612,7 +632,7 @@ extern "C" int spm_int_exact(const int64_t* a, int n, int device, void* stream, 
     if (!out) return SPM_EINVAL;
     const double t0 = now_seconds();
     spm_plan* p = nullptr;
-    int st = make_int_plan(a, n, device, &p);
+    int st = make_int_plan(a, n, SPM_INT_WALK_AUTO, device, &p);
     if (st != SPM_OK) {
         std::memset(out, 0, sizeof *out);
         out->status = st;

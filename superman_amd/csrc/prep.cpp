@@ -73,6 +73,30 @@ void int_planes(const int64_t* a, int n, int* planes)
         }
 }
 
+// True when every absolute row sum is below 2^int_small_bits(n), so that
+// the FP64 integer walk (int_small_kernel, batch kind 4) is exact.
+bool int_small_ok(const int64_t* a, int n)
+{
+    const int64_t lim = int64_t(1) << int_small_bits(n);
+    for (int i = 0; i < n; ++i) {
+        int64_t ra = 0;
+        for (int j = 0; j < n; ++j) ra += a[i * n + j] < 0 ? -a[i * n + j] : a[i * n + j];
+        if (ra >= lim) return false;
+    }
+    return true;
+}
+
+// +2A and -2A, column-major doubles (2 n^2 values).
+void int_small_planes(const int64_t* a, int n, double* planes)
+{
+    const size_t nn = (size_t)n * n;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            planes[(size_t)j * n + i] = (double)(2 * a[i * n + j]);
+            planes[nn + (size_t)j * n + i] = (double)(-2 * a[i * n + j]);
+        }
+}
+
 // rec = (hi, lo, abs) for real, (re hi, re lo, abs, im hi, im lo) for
 // complex: the compensated sum over all 2^(n-1) terms.
 void dense_fill_result(int n, bool complex_kind, bool compensated, const double* rec, spm_result* r)

@@ -9,7 +9,8 @@ namespace spm {
 constexpr int kBatchMaxN = 28;      // orders handled by the batched kernel
 constexpr int kBatchReal = 1;       // compensated FP64
 constexpr int kBatchComplex = 2;    // compensated complex128
-constexpr int kBatchInt = 3;        // exact integer (mod 2^128)
+constexpr int kBatchInt = 3;        // exact integer (mod 2^128), int32 walk
+constexpr int kBatchIntSmall = 4;   // exact integer, small row sums: FP64 walk
 constexpr int kBatchRec = 8;        // doubles per item / result record
 constexpr int kBatchAlign = 64;     // byte alignment of every arena region
 
@@ -18,8 +19,8 @@ struct BatchHeader {                // 64 bytes at arena offset 0
     uint32_t items;
     uint64_t desc_off;              // BatchRequestDesc[requests]
     uint64_t item_off;              // uint32_t[items]: item -> request
-    uint32_t range_begin[4];        // item range walked by kernel q: q = 0 all
-    uint32_t range_end[4];          // kinds (fused kernel), q = kind (one-kind kernels)
+    uint32_t range_begin[4];        // item range walked by kernel q: q = 0 all kinds
+    uint32_t range_end[4];          // (fused kernel); q = 1 real, 2 complex, 3 integer
     uint64_t pad;
 };
 
@@ -39,12 +40,12 @@ static_assert(sizeof(BatchHeader) == 64, "header layout");
 static_assert(sizeof(BatchRequestDesc) == 64, "descriptor layout");
 
 // batch.hip
-//   walk:     (const uint8_t* arena, double* item_rec, unsigned int* counter);
-//             `kinds` = 7 (every kind in one kernel) or 1, 2, 4 (one kind)
-//   finalize: (const uint8_t* arena, const double* item_rec, double* result_rec)
-//   reset:    (unsigned int* counter)
+//   walk:  (const uint8_t* arena, double* item_rec, unsigned int* counter /* [4] */,
+//           unsigned int* done /* [requests] */, double* result_rec);
+//          `kinds` = 15 (every kind in one kernel), 1 (real), 2 (complex)
+//          or 12 (both integer walks)
+//   reset: (unsigned int* counter, unsigned int* done, unsigned int n_done)
 const void* batch_walk_kernel_ptr(int kinds);
-const void* batch_finalize_kernel_ptr();
 const void* batch_reset_kernel_ptr();
 
 }  // namespace spm

@@ -44,6 +44,8 @@ inline double complex_err_est(int n, double abs_sum)
 void dense_prepare(const double* a, int n, int w, double* planes, double* x0);
 int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound);
 void int_planes(const int64_t* a, int n, int* planes);
+bool int_small_ok(const int64_t* a, int n);
+void int_small_planes(const int64_t* a, int n, double* planes);
 void dense_fill_result(int n, bool complex_kind, bool compensated, const double* rec, spm_result* r);
 int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r);
 

@@ -34,6 +34,13 @@ const void* int_dense_kernel_ptr(int n);
 //   args: (const unsigned long long* partial, int nrec, unsigned long long* out /* [2] */)
 const void* reduce_i128_kernel_ptr();
 
+// int_small.hip -- exact walk in FP64 for integer matrices whose absolute
+// row sums are all below 2^int_small_bits(n) (every 0/1 matrix); same
+// arguments as int_dense_kernel except that M is [2][n][n] doubles (+2A,
+// -2A) and y0 is n doubles.
+const void* int_small_kernel_ptr(int n);
+int int_small_bits(int n);
+
 // Generated SpaRyser kernels walk 2^3-step blocks with L > 3.
 constexpr int kCodegenMinN = 5;
 

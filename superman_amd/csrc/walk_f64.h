@@ -214,16 +214,40 @@ __device__ __forceinline__ void fma_column(double (&x)[N], const Column<N>& c, d
     }
 }
 
-// One lane's chunk: the 2^L Gray steps starting at k = h 2^L.  Returns the
-// chunk's term sum (t_hi + t_lo) and sum |term| (t_abs, COMP only).
-template <int N, int U, bool COMP>
-__device__ __forceinline__ void walk_f64(const double* __restrict__ M,   // [2][N][N]: +A, -A
-                                         const double* __restrict__ x0,  // x(empty set)
-                                         uint64_t h, int L, double& t_hi, double& t_lo, double& t_abs)
+// The accumulation policy of a walk: add(x, negative) takes the term
+// (-1)^k prod_i x_i of the current subset, fold() is called every
+// kFoldSteps steps and at the end of the chunk.
+//
+// Compensated/plain FP64 sums: block sums in plain FP64 for kFoldSteps
+// steps, folded into the per-lane (compensated) sum.
+template <int N, bool COMP>
+struct FpTerm {
+    double t_hi = 0.0, t_lo = 0.0, t_abs = 0.0;
+    double blk = 0.0, blk_abs = 0.0;
+    __device__ __forceinline__ void add(const double (&x)[N], bool negative)
+    {
+        add_term<N, COMP>(x, negative, blk, blk_abs);
+    }
+    __device__ __forceinline__ void fold()
+    {
+        if constexpr (COMP) {
+            dd_add_d(t_hi, t_lo, blk);
+            t_abs += blk_abs;
+        } else {
+            t_hi += blk;
+        }
+        blk = 0.0;
+        blk_abs = 0.0;
+    }
+};
+
+// One lane's chunk: the 2^L Gray steps starting at k = h 2^L, every term
+// handed to `term`.
+template <int N, int U, class Term>
+__device__ __forceinline__ void walk_gray(const double* __restrict__ M,   // [2][N][N]: +A, -A
+                                          const double* __restrict__ x0,  // x(empty set)
+                                          uint64_t h, int L, Term& term)
 {
-    t_hi = 0.0;
-    t_lo = 0.0;
-    t_abs = 0.0;
     // x at the chunk start G(h 2^L): columns >= L-1 may be set.
     // (Loads go through step_column too: kept as loop invariants, x0 alone
     // would pin 2N SGPRs for the whole task loop.)
@@ -244,19 +268,7 @@ __device__ __forceinline__ void walk_f64(const double* __restrict__ M,   // [2][
 
     const double lane_sign = (h & 1u) ? -1.0 : 1.0;  // direction of column L-1
     const uint32_t nblk = 1u << (L - U);
-    double blk = 0.0, blk_abs = 0.0;
-    // Block sums accumulate in plain FP64 for kFoldSteps steps, then fold
-    // into the compensated per-lane sum.
-    auto fold = [&]() __attribute__((always_inline)) {
-        if constexpr (COMP) {
-            dd_add_d(t_hi, t_lo, blk);
-            t_abs += blk_abs;
-        } else {
-            t_hi += blk;
-        }
-        blk = 0.0;
-        blk_abs = 0.0;
-    };
+    auto fold = [&]() __attribute__((always_inline)) { term.fold(); };
     if constexpr (U == 1) {
         // Blocks of 2 steps: k = 2m flips column ctz(m)+1, k = 2m+1 flips
         // column 0.  A uniform direction selects the +A or -A plane, so a
@@ -274,12 +286,12 @@ __device__ __forceinline__ void walk_f64(const double* __restrict__ M,   // [2][
         const uint32_t half = nblk >> 1;
         constexpr uint32_t G = kFoldSteps / 2;
         Column<N> ca, cb;
-        add_term<N, COMP>(x, false, blk, blk_abs);  // k = 0: the chunk start
+        term.add(x, false);  // k = 0: the chunk start
         load_column(cb, M);                          // k = 1: +column 0
         add_column(x, cb);
         load_column(ca, a_col(1));
         step_fence();
-        add_term<N, COMP>(x, true, blk, blk_abs);
+        term.add(x, true);
         step_fence();
         // One block; SPECIAL marks block `half`, whose column L-1 flips in a
         // per-lane direction.  Kept branch-free: the loop is split there.
@@ -288,12 +300,12 @@ __device__ __forceinline__ void walk_f64(const double* __restrict__ M,   // [2][
             else add_column(x, ca);
             load_column(cb, (m & 1u) ? Mneg : M);  // step 2m+1: column 0
             step_fence();
-            add_term<N, COMP>(x, false, blk, blk_abs);
+            term.add(x, false);
             step_fence();
             add_column(x, cb);
             load_column(ca, a_col(m + 1));  // m+1 = nblk loads column L <= N-1: harmless
             step_fence();
-            add_term<N, COMP>(x, true, blk, blk_abs);
+            term.add(x, true);
             step_fence();
         };
         auto run = [&](uint32_t m, uint32_t end) __attribute__((always_inline)) {
@@ -327,11 +339,11 @@ __device__ __forceinline__ void walk_f64(const double* __restrict__ M,   // [2][
                     } else {
                         flip_signed<N>(x, M + c * N, s_mid);
                     }
-                    add_term<N, COMP>(x, (j & 1) != 0, blk, blk_abs);
+                    term.add(x, (j & 1) != 0);
                 });
             }
         };
-        add_term<N, COMP>(x, false, blk, blk_abs);  // block 0: the chunk start
+        term.add(x, false);  // block 0: the chunk start
         block_tail(0u);
         uint32_t m = 1;
         while (m < nblk) {
@@ -342,13 +354,26 @@ __device__ __forceinline__ void walk_f64(const double* __restrict__ M,   // [2][
                 const int c = tz + U;
                 const double s = (c + 1 < L) ? (((m >> (tz + 1)) & 1u) ? -1.0 : 1.0) : lane_sign;
                 flip_signed<N>(x, M + c * N, s);
-                add_term<N, COMP>(x, U == 0 && (m & 1u), blk, blk_abs);
+                term.add(x, U == 0 && (m & 1u));
                 block_tail(m);
             }
             fold();
         }
         fold();
     }
+}
+
+// The compensated (COMP) or plain FP64 sum of one lane's chunk: t_hi + t_lo
+// and sum |term| (t_abs, COMP only).
+template <int N, int U, bool COMP>
+__device__ __forceinline__ void walk_f64(const double* __restrict__ M, const double* __restrict__ x0, uint64_t h,
+                                         int L, double& t_hi, double& t_lo, double& t_abs)
+{
+    FpTerm<N, COMP> term;
+    walk_gray<N, U>(M, x0, h, L, term);
+    t_hi = term.t_hi;
+    t_lo = term.t_lo;
+    t_abs = term.t_abs;
 }
 
 }  // namespace real

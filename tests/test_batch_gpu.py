@@ -59,6 +59,18 @@ def test_every_order_vs_oracle(engine, kind):
         check_vs_oracle(k, a, p)
 
 
+def test_both_integer_walks_in_one_batch(engine):
+    # 0/1 matrices take the FP64 walk, wider entries the int32 walk
+    rng = np.random.default_rng(77)
+    reqs = []
+    for n in range(4, 19):
+        reqs.append(("int", rng.integers(-3, 4, (n, n)).astype(np.float64)))
+        reqs.append(("int", (rng.random((n, n)) < 0.5).astype(np.float64)))
+    res = engine.run(reqs)
+    for (k, a), p in zip(reqs, res):
+        check_vs_oracle(k, a, p)
+
+
 def test_large_orders_match_single_matrix_path(engine):
     reqs = []
     for n in range(23, 29):

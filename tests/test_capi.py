@@ -69,3 +69,12 @@ def test_python_api_raises_without_device():
         pytest.skip("a GPU is visible")
     with pytest.raises(_lib.SpmError):
         S.permanent(np.ones((3, 3)))
+
+
+def test_int_walk_selection_rejected_without_gpu():
+    L = _lib.lib()
+    plan = ctypes.c_void_p()
+    a = np.full((6, 6), 9, dtype=np.int64)  # row sums 54 >= 2^3: the FP64 walk does not apply
+    ap = a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    assert L.spm_plan_int_walk(ap, 6, _lib.SPM_INT_WALK_FP64, 0, ctypes.byref(plan)) == _lib.SPM_ERANGE
+    assert L.spm_plan_int_walk(ap, 6, 7, 0, ctypes.byref(plan)) == _lib.SPM_EINVAL

@@ -94,3 +94,36 @@ def test_auto_dispatch_uses_exact_path():
     a = (np.random.default_rng(24).random((24, 24)) < 0.5).astype(np.float64)
     r = api.permanent(a)
     assert r.exact is not None and r.exact == O.int_exact(a.astype(np.int64))[0]
+
+
+def _walk(a, walk):
+    with api.DensePlan(np.asarray(a, dtype=np.int64), "exact", int_walk=walk) as p:
+        return p.run().exact, p.info()["kernel"]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 13, 17, 22, 24, 26, 28])
+def test_int_walks_agree(n):
+    # entries in {-1, 0, 1}: absolute row sums <= n < 2^(bit width of n),
+    # so both the int32 walk and the FP64 walk apply; about 5 nonzeros per
+    # row keeps 2^(n-1) prod_i sum_j |a_ij| below the 2^126 exactness bound
+    rng = np.random.default_rng(900 + n)
+    a = rng.choice([-1, 1], (n, n)) * (rng.random((n, n)) < min(1.0, 5.0 / n))
+    ref, st = O.int_exact(a)
+    assert st == 0
+    v32, k32 = _walk(a, "int32")
+    v64, k64 = _walk(a, "fp64")
+    assert k32.startswith("int_dense_kernel") and k64.startswith("int_small_kernel")
+    assert v32 == v64 == ref, (n, v32, v64, ref)
+
+
+@pytest.mark.parametrize("case", [c for c in load_golden() if c["name"].startswith(("int01_d50", "sparse01_n2"))],
+                         ids=lambda c: c["name"])
+def test_int_walks_agree_golden(case):
+    a = golden_matrix(case)
+    assert _walk(a, "int32")[0] == _walk(a, "fp64")[0] == int(case["exact"])
+
+
+def test_fp64_walk_sparse01_n40_matches_reference(golden):
+    c = golden["c3_sparse01_n40_s40"]
+    v, k = _walk(golden_matrix(c), "fp64")
+    assert k == "int_small_kernel<40>" and v == int(c["ref8"])
