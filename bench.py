@@ -67,6 +67,19 @@ def cpu_baseline(a: np.ndarray, target_s: float = 12.0) -> dict:
                       f"{rate:.3e} subsets/s scaled to 2^{n - 1} subsets per permanent"}
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch from the committed counter runs (two rocprofv3
+    --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known
+    size), when they were taken on this kernel; else None."""
+    path = os.path.join(ROOT, "profiles", "dense_f64_n40_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    return t["bytes_per_launch"] if t.get("kernel") == kernel else None
+
+
 def serving(count: int, seed: int = 2026, load: float = 0.8) -> dict:
     from superman_amd import api
     from superman_amd import workloads as W
@@ -218,7 +231,7 @@ def main(argv=None):
                                f"accumulation), 2^{n - 1} Gray-code subsets",
                    "n": n, "precision": args.precision, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(info["kernel"]),
                      "kernel": info["kernel"], "kernel_ms": k_ms,
                      "flops_per_launch": flops, "flop_convention": "2n per subset (add, mul, accumulate; FMA = 2)"},
         "result": {"value": r.value, "err_est": r.err_est, "cond": r.cond,

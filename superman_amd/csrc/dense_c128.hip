@@ -16,8 +16,21 @@ namespace spm {
 
 namespace cplx {
 
+// Minimum waves per SIMD asked of the register allocator, per order: the
+// orders where that occupancy keeps the 2-step walk free of spills (the
+// one-off chunk-start code may spill a few values); 1 = compiler's choice.
+// SPM_C128_WAVES_PER_EU (sweeps only) overrides it for every order.
+#ifndef SPM_C128_WAVES_PER_EU
+#define SPM_C128_WAVES_PER_EU 0
+#endif
+template <int N>
+constexpr int kC128Waves = SPM_C128_WAVES_PER_EU > 0 ? SPM_C128_WAVES_PER_EU
+                           : (N >= 29 && N <= 36)    ? 3
+                           : (N >= 21 && N <= 25)    ? 4
+                                                     : 1;
+
 template <int N, bool COMP>
-__global__ void __launch_bounds__(kDenseBlock)
+__global__ void __launch_bounds__(kDenseBlock) __attribute__((amdgpu_waves_per_eu(kC128Waves<N>)))
 dense_c128_kernel(const double* __restrict__ M,   // [2][N][2N]: +A then -A, column-major, (re, im)
                   const double* __restrict__ x0,  // 2N doubles
                   double* __restrict__ partial,   // [gridDim.x][8]: re hi lo abs, im hi lo 0, 0, 0

@@ -11,7 +11,7 @@
 namespace spm {
 
 template <int N, int U, bool COMP>
-__global__ void __launch_bounds__(kDenseBlock) SPM_DENSE_ATTR
+__global__ void __launch_bounds__(kDenseBlock) __attribute__((amdgpu_waves_per_eu(real::kDenseWaves<N>)))
 dense_f64_kernel(const double* __restrict__ M,   // [2][N][N]: +A then -A, column-major
                  const double* __restrict__ x0,  // x(empty set), N values
                  double* __restrict__ partial,   // [gridDim.x][4]: hi, lo, abs, 0

@@ -44,9 +44,10 @@
 
 #include <type_traits>
 
-// Tuning knobs (build-time).  SPM_DENSE_WAVES_PER_EU caps VGPRs so that this
-// many waves fit per SIMD (0 = compiler's choice); SPM_DENSE_UNROLL is U, the
-// log2 of the steps per unrolled block.
+// Tuning knobs (build-time).  SPM_DENSE_WAVES_PER_EU (sweeps only) caps
+// VGPRs so that this many waves fit per SIMD for every order (0 = the
+// per-order table kDenseWaves); SPM_DENSE_UNROLL is U, the log2 of the steps
+// per unrolled block.
 #ifndef SPM_DENSE_WAVES_PER_EU
 #define SPM_DENSE_WAVES_PER_EU 0
 #endif
@@ -60,11 +61,6 @@
 // SPM_DENSE_ONLY_N (sweeps only): instantiate a single order.
 #ifndef SPM_DENSE_ONLY_N
 #define SPM_DENSE_ONLY_N 0
-#endif
-#if SPM_DENSE_WAVES_PER_EU > 0
-#define SPM_DENSE_ATTR __attribute__((amdgpu_waves_per_eu(SPM_DENSE_WAVES_PER_EU)))
-#else
-#define SPM_DENSE_ATTR
 #endif
 
 namespace spm {
@@ -160,6 +156,12 @@ __device__ __forceinline__ void step_fence()
 #endif
 template <int N>
 constexpr int kColumnHead = N <= SPM_DENSE_PREFETCH_MAX ? N : 32;
+
+// Minimum waves per SIMD asked of the register allocator: 5 for the orders
+// 29..40, whose natural allocation (97..119 VGPRs) leaves 4, and where 96
+// VGPRs keep the 2-step walk free of spills; 1 = compiler's choice.
+template <int N>
+constexpr int kDenseWaves = SPM_DENSE_WAVES_PER_EU > 0 ? SPM_DENSE_WAVES_PER_EU : (N >= 29 && N <= 40) ? 5 : 1;
 
 // Unroll depth per order: the pipelined 2-step blocks up to the order whose
 // prefetched column still fits the SGPR file; single steps beyond that and
