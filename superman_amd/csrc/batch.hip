@@ -12,13 +12,16 @@
 // and order (wave-uniform) and writes the item's wave-reduced sum to its own
 // record.  A wave leaves when the counter passes the item count, so every
 // wave reaches the exit.  The wave that completes a matrix's last item (a
-// per-matrix count) reduces that matrix's item records in a fixed order:
-// the result of a matrix depends only on its data and its chunk size 2^L,
-// never on the batch it rides in or on scheduling.
+// per-matrix count) reduces that matrix's item records in a fixed order and
+// publishes the result to host memory: the result of a matrix depends only
+// on its data and its chunk size 2^L, never on the batch it rides in or on
+// scheduling.
 //
 // Arena layout (byte offsets in the header, all 64-byte aligned):
 //   header | request descriptors | item -> request table | matrix data
-// Records: item_rec[item][8], result_rec[request][8] (doubles):
+// Records: item_rec[item][8] (device), result_host[request][8] (doubles, in
+// host-mapped memory, published with a per-request flag so the host can
+// hand a result out as soon as its last item is done):
 //   real    : hi, lo, abs
 //   complex : re hi, re lo, abs, im hi, im lo
 //   integer : T mod 2^128 as (lo, hi) 64-bit words (both integer walks:
@@ -182,11 +185,13 @@ __device__ __forceinline__ void reduce_request(int kind, const double* __restric
 template <int KINDS>
 __global__ void __launch_bounds__(kDenseBlock)
 batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_rec, unsigned int* __restrict__ counter,
-                  unsigned int* __restrict__ done, double* __restrict__ result_rec)
+                  unsigned int* __restrict__ done, double* __restrict__ result_host,
+                  unsigned int* __restrict__ flag_host)
 {
     using namespace batch;
     const BatchHeader* hd = (const BatchHeader*)arena;
     constexpr int q = KINDS == 15 ? 0 : KINDS == 1 ? 1 : KINDS == 2 ? 2 : 3;
+    const uint32_t seq = uniform(hd->seq);
     const uint32_t begin = uniform(hd->range_begin[q]);
     const uint32_t end = uniform(hd->range_end[q]);
     const BatchRequestDesc* reqs = (const BatchRequestDesc*)(arena + hd->desc_off);
@@ -231,7 +236,10 @@ batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_r
         if (prev + 1 == items) {
             __threadfence();
             reduce_request(kind, item_rec + (size_t)uniform(d->item_base) * kBatchRec, items, lane,
-                           result_rec + (size_t)r * kBatchRec);
+                           result_host + (size_t)r * kBatchRec);
+            // publish: the record reaches host memory before the flag does
+            __threadfence_system();
+            flag_host[r] = seq;
         }
         i = begin + uniform(atomicAdd(counter + q, lane == 0 ? 1u : 0u));
     }

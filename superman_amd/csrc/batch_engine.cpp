@@ -5,7 +5,11 @@
 // pinned staging buffer, a device arena (header, request descriptors, item
 // table, matrix data), item and result records, the work counters and,
 // optionally, a captured hipGraph of the batch's launch sequence
-//   reset counters -> walk kernel(s) -> results to pinned host.
+//   reset counters -> walk kernel(s).
+// Each request's result is published by the walk kernel itself into
+// host-mapped memory followed by a per-request flag, so progress() hands a
+// result out as soon as its flag shows, without waiting for the rest of the
+// batch; the batch's completion event only frees the slot.
 // Requests are prepared on the host at submission (x(empty), +/- planes;
 // integer checks), queue in FIFO order, and are admitted into the next batch
 // while it stays within the slot's request, item, byte and work budgets.
@@ -60,7 +64,7 @@ struct Req {
     int L = 0;
     uint64_t chunks = 0;
     uint32_t items = 0;
-    double item_cost = 0, work = 0;
+    double work = 0;
     double t_submit = 0, t_deadline = 0, t_launch = 0;
     bool cancelled = false;
     int index = -1;             // position in a run/replay, -1 for served requests
@@ -74,8 +78,12 @@ struct Slot {
     uint8_t* h_stage = nullptr;
     uint8_t* d_arena = nullptr;
     double* d_item = nullptr;
-    double* d_res = nullptr;
-    double* h_res = nullptr;
+    double* h_res = nullptr;        // host-mapped result records [max_requests][8]
+    unsigned int* h_flag = nullptr; // host-mapped flags [max_requests]: seq when published
+    double* dh_res = nullptr;       // device addresses of the two
+    unsigned int* dh_flag = nullptr;
+    uint32_t seq = 0;               // batch number of the slot's current batch
+    std::vector<char> taken;        // result already handed out (per staged request)
     unsigned int* d_counter = nullptr;  // [4] work counters, then [max_requests] item counts
     std::vector<Req*> reqs;
     bool busy = false;
@@ -123,7 +131,6 @@ void geometry(const spm_batch* b, Req* r)
     r->chunks = uint64_t(1) << (B - L);
     r->items = (uint32_t)((r->chunks + kWaveLanes - 1) / kWaveLanes);
     const double ops = ops_per_subset(r->dev_kind, r->n);
-    r->item_cost = std::ldexp(1.0, L) * kWaveLanes * ops + double(r->n) * r->n;
     r->work = std::ldexp(1.0, B) * ops;
 }
 
@@ -225,9 +232,9 @@ void free_slot(Slot& s)
     if (s.ev1) (void)hipEventDestroy(s.ev1);
     if (s.h_stage) (void)hipHostFree(s.h_stage);
     if (s.h_res) (void)hipHostFree(s.h_res);
+    if (s.h_flag) (void)hipHostFree(s.h_flag);
     if (s.d_arena) (void)hipFree(s.d_arena);
     if (s.d_item) (void)hipFree(s.d_item);
-    if (s.d_res) (void)hipFree(s.d_res);
     if (s.d_counter) (void)hipFree(s.d_counter);
     s = Slot();
 }
@@ -239,10 +246,10 @@ int grid_for(const spm_batch* b, int q)
     return std::max(1, b->cus * wpc / (kDenseBlock / kWaveLanes));
 }
 
-// The launch sequence of one batch after its arena upload; `nreq` sizes the
-// result copy (the graph uses the slot maximum).  Each request's result is
-// reduced inside the walk kernel by the wave that finishes its last item.
-hipError_t enqueue(spm_batch* b, Slot& s, int nreq)
+// The launch sequence of one batch after its arena upload.  Each request's
+// result is reduced inside the walk kernel by the wave that finishes its last
+// item and published to the slot's host-mapped records.
+hipError_t enqueue(spm_batch* b, Slot& s)
 {
     hipError_t e;
     unsigned int* done = s.d_counter + 64;
@@ -257,11 +264,11 @@ hipError_t enqueue(spm_batch* b, Slot& s, int nreq)
     for (int k = 0; k < (b->tune.fused ? 1 : 3); ++k) {
         const int mask = b->tune.fused ? 15 : kinds[k];
         const int q = b->tune.fused ? 0 : k + 1;
-        void* args[] = {&s.d_arena, &s.d_item, &s.d_counter, &done, &s.d_res};
+        void* args[] = {&s.d_arena, &s.d_item, &s.d_counter, &done, &s.dh_res, &s.dh_flag};
         e = hipLaunchKernel(batch_walk_kernel_ptr(mask), dim3(grid_for(b, q)), dim3(kDenseBlock), args, 0, s.stream);
         if (e != hipSuccess) return e;
     }
-    return hipMemcpyAsync(s.h_res, s.d_res, sizeof(double) * kBatchRec * nreq, hipMemcpyDeviceToHost, s.stream);
+    return hipSuccess;
 }
 
 hipError_t capture(spm_batch* b, Slot& s)
@@ -269,7 +276,7 @@ hipError_t capture(spm_batch* b, Slot& s)
     free_graph(s);
     hipError_t e = hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return e;
-    hipError_t e1 = enqueue(b, s, b->cfg.max_requests);
+    hipError_t e1 = enqueue(b, s);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s.stream, &g);
     if (e1 != hipSuccess) {
@@ -319,22 +326,25 @@ std::vector<Req*> admit(spm_batch* b, double now)
 }
 
 // Writes header, descriptors, item table and data into the slot's staging
-// buffer; returns the bytes to upload.  Item order: fused -- every item by
-// decreasing cost; per kind -- grouped by kind, decreasing cost inside.
+// buffer; returns the bytes to upload.  Item order: shortest request first
+// (fused: over all kinds; per kind: grouped by kernel), so that small
+// requests publish their results early; a large request's items are all of
+// one size, so the batch's tail is at most one item long.
 size_t stage(spm_batch* b, Slot& s, std::vector<Req*>& rs)
 {
     if (b->tune.fused) {
-        std::stable_sort(rs.begin(), rs.end(), [](const Req* x, const Req* y) { return x->item_cost > y->item_cost; });
+        std::stable_sort(rs.begin(), rs.end(), [](const Req* x, const Req* y) { return x->work < y->work; });
     } else {
         std::stable_sort(rs.begin(), rs.end(), [](const Req* x, const Req* y) {
             const int gx = walk_group(x->dev_kind), gy = walk_group(y->dev_kind);
-            return gx != gy ? gx < gy : x->item_cost > y->item_cost;
+            return gx != gy ? gx < gy : x->work < y->work;
         });
     }
     uint8_t* h = s.h_stage;
     BatchHeader hd;
     std::memset(&hd, 0, sizeof hd);
     hd.requests = (uint32_t)rs.size();
+    hd.seq = s.seq;
     hd.desc_off = b->desc_off;
     hd.item_off = b->item_off;
     BatchRequestDesc* desc = (BatchRequestDesc*)(h + b->desc_off);
@@ -390,8 +400,10 @@ int launch(spm_batch* b, Slot& s, std::vector<Req*>& rs, double now)
         s.t_ready = now + 20e-6 + work / 2e13;
         return SPM_OK;
     }
+    ++s.seq;
     const size_t bytes = stage(b, s, rs);
     s.reqs = rs;  // staged order = record order
+    s.taken.assign(rs.size(), 0);
     DeviceGuard g(b->cfg.device);
     if (!g.ok()) return SPM_EHIP;
     if (hipEventRecord(s.ev0, s.stream) != hipSuccess) return SPM_EHIP;
@@ -399,7 +411,7 @@ int launch(spm_batch* b, Slot& s, std::vector<Req*>& rs, double now)
     if (b->cfg.use_graph) {
         if (!s.exec && capture(b, s) != hipSuccess) return SPM_EHIP;
         if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return SPM_EHIP;
-    } else if (enqueue(b, s, (int)rs.size()) != hipSuccess) {
+    } else if (enqueue(b, s) != hipSuccess) {
         return SPM_EHIP;
     }
     if (hipEventRecord(s.ev1, s.stream) != hipSuccess) return SPM_EHIP;
@@ -408,7 +420,8 @@ int launch(spm_batch* b, Slot& s, std::vector<Req*>& rs, double now)
 
 void fail_slot(spm_batch* b, Slot& s, int status, double now)
 {
-    for (Req* r : s.reqs) complete(b, r, status, nullptr, now, 0.f);
+    for (Req* r : s.reqs)
+        if (r) complete(b, r, status, nullptr, now, 0.f);
     s.reqs.clear();
     s.busy = false;
 }
@@ -432,17 +445,29 @@ int retire(spm_batch* b, double now)
             ++got;
             continue;
         }
+        // results already published by the kernel
+        for (size_t j = 0; j < s.reqs.size(); ++j) {
+            if (s.taken[j] || __atomic_load_n(s.h_flag + j, __ATOMIC_ACQUIRE) != s.seq) continue;
+            s.taken[j] = 1;
+            complete(b, s.reqs[j], SPM_OK, s.h_res + j * kBatchRec, now, -1.f);
+            s.reqs[j] = nullptr;
+        }
         const hipError_t e = hipEventQuery(s.ev1);
         if (e == hipErrorNotReady) continue;
         if (e != hipSuccess) {
-            fail_slot(b, s, SPM_EHIP, now);
+            for (size_t j = 0; j < s.reqs.size(); ++j)
+                if (!s.taken[j]) complete(b, s.reqs[j], SPM_EHIP, nullptr, now, 0.f);
+            s.reqs.clear();
+            s.busy = false;
             ++got;
             continue;
         }
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, s.ev0, s.ev1) != hipSuccess) ms = -1.f;
-        for (size_t j = 0; j < s.reqs.size(); ++j)
-            complete(b, s.reqs[j], SPM_OK, s.h_res + j * kBatchRec, now, ms);
+        // the batch is over: every flag must be set by now
+        for (size_t j = 0; j < s.reqs.size(); ++j) {
+            if (s.taken[j]) continue;
+            const bool ok = __atomic_load_n(s.h_flag + j, __ATOMIC_ACQUIRE) == s.seq;
+            complete(b, s.reqs[j], ok ? SPM_OK : SPM_EHIP, ok ? s.h_res + j * kBatchRec : nullptr, now, -1.f);
+        }
         s.reqs.clear();
         s.busy = false;
         ++got;
@@ -655,15 +680,22 @@ extern "C" int spm_batch_create(const spm_batch_config* cfg, spm_batch** out)
             return SPM_EHIP;
         }
         ok = hipHostMalloc(&s.h_stage, b->stage_bytes, hipHostMallocDefault) == hipSuccess &&
-             hipHostMalloc(&s.h_res, sizeof(double) * kBatchRec * cfg->max_requests, hipHostMallocDefault) ==
-                 hipSuccess &&
+             hipHostMalloc(&s.h_res, sizeof(double) * kBatchRec * cfg->max_requests,
+                           hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
+             hipHostMalloc(&s.h_flag, sizeof(unsigned int) * cfg->max_requests,
+                           hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
              hipMalloc(&s.d_arena, b->stage_bytes) == hipSuccess &&
              hipMalloc(&s.d_item, sizeof(double) * kBatchRec * (size_t)cfg->max_items) == hipSuccess &&
-             hipMalloc(&s.d_res, sizeof(double) * kBatchRec * cfg->max_requests) == hipSuccess &&
              hipMalloc(&s.d_counter, sizeof(unsigned int) * (64 + (size_t)cfg->max_requests)) == hipSuccess;
         if (!ok) {
             for (Slot& t : b->slots) free_slot(t);
             return SPM_ENOMEM;
+        }
+        std::memset(s.h_flag, 0, sizeof(unsigned int) * cfg->max_requests);
+        if (hipHostGetDevicePointer((void**)&s.dh_res, s.h_res, 0) != hipSuccess ||
+            hipHostGetDevicePointer((void**)&s.dh_flag, s.h_flag, 0) != hipSuccess) {
+            for (Slot& t : b->slots) free_slot(t);
+            return SPM_EHIP;
         }
     }
     *out = b.release();
@@ -682,7 +714,7 @@ extern "C" int spm_batch_destroy(spm_batch* b)
             for (Slot& s : b->slots) free_slot(s);
         }
         for (Slot& s : b->slots)
-            for (Req* r : s.reqs) delete r;
+            for (Req* r : s.reqs) delete r;  // (null entries were handed out)
         for (Req* r : b->pending) delete r;
     }
     delete b;
@@ -839,7 +871,9 @@ extern "C" int spm_serve_pending(spm_batch* b, int* queued, int* in_flight)
     if (queued) *queued = (int)b->pending.size();
     if (in_flight) {
         int k = 0;
-        for (const Slot& s : b->slots) k += s.busy ? (int)s.reqs.size() : 0;
+        for (const Slot& s : b->slots)
+            if (s.busy)
+                for (const Req* r : s.reqs) k += r ? 1 : 0;
         *in_flight = k;
     }
     return SPM_OK;

@@ -21,7 +21,8 @@ struct BatchHeader {                // 64 bytes at arena offset 0
     uint64_t item_off;              // uint32_t[items]: item -> request
     uint32_t range_begin[4];        // item range walked by kernel q: q = 0 all kinds
     uint32_t range_end[4];          // (fused kernel); q = 1 real, 2 complex, 3 integer
-    uint64_t pad;
+    uint32_t seq;                   // batch number, written to a request's flag when done
+    uint32_t pad;
 };
 
 struct BatchRequestDesc {           // 64 bytes
@@ -41,7 +42,10 @@ static_assert(sizeof(BatchRequestDesc) == 64, "descriptor layout");
 
 // batch.hip
 //   walk:  (const uint8_t* arena, double* item_rec, unsigned int* counter /* [4] */,
-//           unsigned int* done /* [requests] */, double* result_rec);
+//           unsigned int* done /* [requests] */, double* result_host /* [requests][8] */,
+//           unsigned int* flag_host /* [requests] */)
+//          result_host / flag_host are host-mapped: a request's result record
+//          is published there, then its flag is set to the header's seq.
 //          `kinds` = 15 (every kind in one kernel), 1 (real), 2 (complex)
 //          or 12 (both integer walks)
 //   reset: (unsigned int* counter, unsigned int* done, unsigned int n_done)
