@@ -584,6 +584,14 @@ int drive(spm_batch* b, int count, const spm_request* reqs, const double* arriva
             if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
         }
     }
+    // Results are handed out before their batch's kernels finish; leave the
+    // engine idle (every slot retired) for the next caller.
+    for (;;) {
+        int busy = 0;
+        progress(b, &busy);
+        if (busy == 0) break;
+        std::this_thread::yield();
+    }
     if (latency_s)
         for (int i = 0; i < count; ++i) latency_s[i] = lat[i];
     if (stats) {

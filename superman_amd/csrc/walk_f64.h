@@ -158,10 +158,12 @@ template <int N>
 constexpr int kColumnHead = N <= SPM_DENSE_PREFETCH_MAX ? N : 32;
 
 // Minimum waves per SIMD asked of the register allocator: 5 for the orders
-// 29..40, whose natural allocation (97..119 VGPRs) leaves 4, and where 96
-// VGPRs keep the 2-step walk free of spills; 1 = compiler's choice.
+// 29..36, whose natural allocation (97..111 VGPRs) leaves 4 and where 96
+// VGPRs keep the 2-step walk free of spills (measured 3-5% faster at n = 30,
+// 32, 36; at n = 40 occupancy 5 measured 1.7% slower than 4); 1 = the
+// compiler's choice.
 template <int N>
-constexpr int kDenseWaves = SPM_DENSE_WAVES_PER_EU > 0 ? SPM_DENSE_WAVES_PER_EU : (N >= 29 && N <= 40) ? 5 : 1;
+constexpr int kDenseWaves = SPM_DENSE_WAVES_PER_EU > 0 ? SPM_DENSE_WAVES_PER_EU : (N >= 29 && N <= 36) ? 5 : 1;
 
 // Unroll depth per order: the pipelined 2-step blocks up to the order whose
 // prefetched column still fits the SGPR file; single steps beyond that and
