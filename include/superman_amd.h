@@ -263,6 +263,9 @@ int spm_serve_replay(spm_batch* b, int count, const spm_request* reqs, const dou
 /* The next `launches` batch launches fail on the host without touching the
  * device; their requests complete with SPM_EFAULTINJ. */
 int spm_batch_inject_fault(spm_batch* b, int launches);
+/* Which HIP call failed last (empty if none); valid until the next call on
+ * the same thread. */
+const char* spm_batch_last_error(spm_batch* b);
 
 #ifdef __cplusplus
 }

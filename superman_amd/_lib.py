@@ -149,6 +149,7 @@ PROTOTYPES = {
     "spm_serve_replay": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(SpmRequest), _D, _R, _D,
                                         ctypes.POINTER(SpmServeStats)]),
     "spm_batch_inject_fault": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "spm_batch_last_error": (ctypes.c_char_p, [_VP]),
 }
 
 

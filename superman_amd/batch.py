@@ -147,8 +147,8 @@ class BatchEngine:
         rq = _Requests(sample)
         best = SpmBatchTuning()
         sec = ctypes.c_double()
-        check(lib().spm_batch_autotune(self._h, rq.count, rq.arr, reps, ctypes.byref(best), ctypes.byref(sec)),
-              "spm_batch_autotune")
+        st = lib().spm_batch_autotune(self._h, rq.count, rq.arr, reps, ctypes.byref(best), ctypes.byref(sec))
+        check(st, "spm_batch_autotune" + (f" ({self.last_error()})" if st == _lib.SPM_EHIP else ""))
         return Tuning.from_c(best), sec.value
 
     # -- synchronous batches -----------------------------------------------
@@ -161,7 +161,7 @@ class BatchEngine:
 
     def run(self, reqs) -> list[Permanent]:
         st, res = self.run_raw(reqs)
-        check(st, "spm_batch_run")
+        check(st, "spm_batch_run" + (f" ({self.last_error()})" if st == _lib.SPM_EHIP else ""))
         return [_perm(r, kind_of(q[0])) for r, q in zip(res, reqs)]
 
     # -- serving ------------------------------------------------------------
@@ -206,6 +206,9 @@ class BatchEngine:
         q, f = ctypes.c_int(), ctypes.c_int()
         check(lib().spm_serve_pending(self._h, ctypes.byref(q), ctypes.byref(f)))
         return q.value, f.value
+
+    def last_error(self) -> str:
+        return lib().spm_batch_last_error(self._h).decode()
 
     def inject_fault(self, launches: int = 1) -> None:
         check(lib().spm_batch_inject_fault(self._h, launches))

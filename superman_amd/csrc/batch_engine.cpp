@@ -31,6 +31,7 @@
 #include <limits>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -110,7 +111,7 @@ struct spm_batch {
     int fault_launches = 0;
     uint64_t batches = 0;
     double busy_since = -1, busy_total = 0;
-    int graphs_valid = 0;
+    std::string last_error;  // the last failed HIP call, for spm_batch_last_error
     std::mutex mu;
 };
 
@@ -406,15 +407,22 @@ int launch(spm_batch* b, Slot& s, std::vector<Req*>& rs, double now)
     s.taken.assign(rs.size(), 0);
     DeviceGuard g(b->cfg.device);
     if (!g.ok()) return SPM_EHIP;
-    if (hipEventRecord(s.ev0, s.stream) != hipSuccess) return SPM_EHIP;
-    if (hipMemcpyAsync(s.d_arena, s.h_stage, bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess) return SPM_EHIP;
-    if (b->cfg.use_graph) {
-        if (!s.exec && capture(b, s) != hipSuccess) return SPM_EHIP;
-        if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return SPM_EHIP;
-    } else if (enqueue(b, s) != hipSuccess) {
+    auto fail = [&](hipError_t e, const char* what) {
+        b->last_error = std::string(what) + ": " + hipGetErrorName(e);
         return SPM_EHIP;
+    };
+    hipError_t e = hipEventRecord(s.ev0, s.stream);
+    if (e != hipSuccess) return fail(e, "hipEventRecord");
+    e = hipMemcpyAsync(s.d_arena, s.h_stage, bytes, hipMemcpyHostToDevice, s.stream);
+    if (e != hipSuccess) return fail(e, "arena upload");
+    if (b->cfg.use_graph) {
+        if (!s.exec && (e = capture(b, s)) != hipSuccess) return fail(e, "graph capture");
+        if ((e = hipGraphLaunch(s.exec, s.stream)) != hipSuccess) return fail(e, "hipGraphLaunch");
+    } else if ((e = enqueue(b, s)) != hipSuccess) {
+        return fail(e, "kernel launch");
     }
-    if (hipEventRecord(s.ev1, s.stream) != hipSuccess) return SPM_EHIP;
+    e = hipEventRecord(s.ev1, s.stream);
+    if (e != hipSuccess) return fail(e, "hipEventRecord");
     return SPM_OK;
 }
 
@@ -455,6 +463,7 @@ int retire(spm_batch* b, double now)
         const hipError_t e = hipEventQuery(s.ev1);
         if (e == hipErrorNotReady) continue;
         if (e != hipSuccess) {
+            b->last_error = std::string("batch execution: ") + hipGetErrorName(e);
             for (size_t j = 0; j < s.reqs.size(); ++j)
                 if (!s.taken[j]) complete(b, s.reqs[j], SPM_EHIP, nullptr, now, 0.f);
             s.reqs.clear();
@@ -784,6 +793,16 @@ extern "C" int spm_batch_autotune(spm_batch* b, int count, const spm_request* sa
         *sec = bestt;
         return SPM_OK;
     };
+    // A candidate whose launches fail on the device side (SPM_EHIP) is
+    // dropped; the starting tuning and the winner must run.
+    auto candidate = [&](const spm_batch_tuning& t, int cnt, const spm_request* rq, double* sec) -> int {
+        const int rc = time_it(t, cnt, rq, sec);
+        if (rc == SPM_EHIP) {
+            *sec = 1e300;
+            return SPM_OK;
+        }
+        return rc;
+    };
     spm_batch_tuning cur = b->tune, win = cur;
     double win_s = 1e300;
     int st = time_it(cur, count, sample, &win_s);
@@ -795,7 +814,7 @@ extern "C" int spm_batch_autotune(spm_batch* b, int count, const spm_request* sa
             t.fused = fused;
             t.waves_per_cu = wpc;
             double s = 0;
-            st = time_it(t, count, sample, &s);
+            st = candidate(t, count, sample, &s);
             if (st != SPM_OK) return st;
             if (s < win_s) {
                 win_s = s;
@@ -818,7 +837,7 @@ extern "C" int spm_batch_autotune(spm_batch* b, int count, const spm_request* sa
             spm_batch_tuning t = win;
             t.log2_chunk[k] = L;
             double s = 0;
-            st = time_it(t, (int)sub.size(), sub.data(), &s);
+            st = candidate(t, (int)sub.size(), sub.data(), &s);
             if (st != SPM_OK) return st;
             if (s < base_s) {
                 base_s = s;
@@ -895,6 +914,15 @@ extern "C" int spm_serve_replay(spm_batch* b, int count, const spm_request* reqs
         if (arrival_s[i] < arrival_s[i - 1]) return SPM_EINVAL;
     std::lock_guard<std::mutex> lk(b->mu);
     return drive(b, count, reqs, arrival_s, out, latency_s, stats);
+}
+
+extern "C" const char* spm_batch_last_error(spm_batch* b)
+{
+    if (!b) return "";
+    std::lock_guard<std::mutex> lk(b->mu);
+    thread_local std::string copy;
+    copy = b->last_error;
+    return copy.c_str();
 }
 
 extern "C" int spm_batch_inject_fault(spm_batch* b, int launches)
