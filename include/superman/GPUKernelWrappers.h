@@ -9,10 +9,12 @@
 // spSingleGPU.cuh, KernelGenSingleGPU.cuh).  Same signatures and Result
 // semantics (time = whole-call wall seconds, permanent = double); a library
 // status other than SPM_OK becomes std::runtime_error, as the reference's
-// configuration errors are (its CUDA errors exit() instead).  The
-// reference's DecomposePerman preprocessing (DM pruning, degree-1/2
-// compression, degree-3/4 splitting) is not applied: it changes the
-// arithmetic, never the permanent.
+// configuration errors are (its CUDA errors exit() instead).  As in the
+// reference, every algorithm runs behind DecomposePerman: blocks larger
+// than 30 are compressed (degree-1/2) and split (degree-3/4) first
+// (spm_permanent_decomposed / spm_decompose); the DM pruning and the
+// component split stay with the caller (IO::readMatrix, main.cpp), as
+// there.
 #ifndef SUPERMAN_AMD_GPUKERNELWRAPPERS_H
 #define SUPERMAN_AMD_GPUKERNELWRAPPERS_H
 
@@ -77,46 +79,33 @@ inline int precision_of(const Settings& s)
 
 }  // namespace superman_amd
 
-// Dense kernels (all dense Algorithm values map to the one dense kernel).
+// DecomposePerman<..., dpSingleGPU / spSingleGPU> (DecomposePerman.h:76-117
+// around dpSingleGPU.cuh:44-219 / spSingleGPU.cuh:44-234): compressions
+// above order 30, then every leaf on the GPU -- exactly for integer leaves
+// (spm_int_exact), else in FP64 with the requested accumulation.  All dense
+// Algorithm values map to the one dense kernel, all sparse ones to the
+// integer walks.
 template <class C, class S>
-class dpSingleGPU : public Permanent<C, S>
+class DecomposedGPU : public Permanent<C, S>
 {
 public:
-    dpSingleGPU(Matrix<S>* matrix, Settings settings) : Permanent<C, S>(matrix, settings) {}
+    DecomposedGPU(Matrix<S>* matrix, Settings settings) : Permanent<C, S>(matrix, settings) {}
     double permanentFunction() override
     {
         const std::vector<double> a = superman_amd::as_double(this->m_Matrix);
-        superman_amd::check(spm_dense_f64(a.data(), this->m_Matrix->nov, superman_amd::precision_of(this->m_Settings),
-                                          this->m_Settings.deviceID, nullptr, &last),
-                            "spm_dense_f64");
+        superman_amd::check(spm_permanent_decomposed(a.data(), this->m_Matrix->nov, SPM_DECOMP_COMPRESS,
+                                                     superman_amd::precision_of(this->m_Settings),
+                                                     this->m_Settings.deviceID, nullptr, &last),
+                            "spm_permanent_decomposed");
         return last.re;
     }
     spm_result last{};
 };
 
-// Generic sparse kernels: integer matrices run exactly, others in FP64.
 template <class C, class S>
-class spSingleGPU : public Permanent<C, S>
-{
-public:
-    spSingleGPU(Matrix<S>* matrix, Settings settings) : Permanent<C, S>(matrix, settings) {}
-    double permanentFunction() override
-    {
-        std::vector<int64_t> ai;
-        const int n = this->m_Matrix->nov;
-        if (superman_amd::integral(this->m_Matrix, ai)) {
-            const int st = spm_int_exact(ai.data(), n, this->m_Settings.deviceID, nullptr, &last);
-            if (st == SPM_OK) return last.re;
-            if (st != SPM_ERANGE) superman_amd::check(st, "spm_int_exact");
-        }
-        const std::vector<double> a = superman_amd::as_double(this->m_Matrix);
-        superman_amd::check(spm_dense_f64(a.data(), n, superman_amd::precision_of(this->m_Settings),
-                                          this->m_Settings.deviceID, nullptr, &last),
-                            "spm_dense_f64");
-        return last.re;
-    }
-    spm_result last{};
-};
+using dpSingleGPU = DecomposedGPU<C, S>;
+template <class C, class S>
+using spSingleGPU = DecomposedGPU<C, S>;
 
 // Generated per-pattern kernels (register-efficient = UT-ordered, or naive).
 template <class C, class S>
@@ -126,15 +115,36 @@ public:
     KernelGenSingleGPU(Matrix<S>* matrix, Settings settings) : Permanent<C, S>(matrix, settings) {}
     double permanentFunction() override
     {
-        std::vector<int64_t> ai;
-        const int n = this->m_Matrix->nov;
-        const int opt = this->m_Settings.algorithm == REGEFFICIENTCODEGENERATION ? SPM_CG_REGISTER_EFFICIENT : 0;
-        if (n >= 5 && superman_amd::integral(this->m_Matrix, ai)) {
-            const int st = spm_sparse_codegen(ai.data(), n, opt, this->m_Settings.deviceID, nullptr, &last);
-            if (st == SPM_OK) return last.re;
-            if (st != SPM_ERANGE) superman_amd::check(st, "spm_sparse_codegen");
+        // The reference compresses degree-1/2 rows and columns above order 30
+        // and generates one kernel for what remains (DecomposePerman.h:124-140).
+        const std::vector<double> a = superman_amd::as_double(this->m_Matrix);
+        spm_decomp* d = nullptr;
+        superman_amd::check(spm_decompose(a.data(), this->m_Matrix->nov, SPM_DECOMP_CODEGEN | SPM_DECOMP_COMPRESS, &d),
+                            "spm_decompose");
+        int leaves = 0, zero = 0, n = 0;
+        spm_decomp_stats(d, nullptr, &leaves, nullptr, nullptr, &zero);
+        if (zero || leaves == 0) {
+            spm_decomp_destroy(d);
+            last = spm_result{};
+            return 0.0;
         }
-        spSingleGPU<C, S> generic(new Matrix<S>(*this->m_Matrix), this->m_Settings);
+        spm_decomp_leaf(d, 0, nullptr, &n, nullptr);
+        Matrix<S>* leaf = new Matrix<S>(n);
+        std::vector<double> b((size_t)n * n);
+        spm_decomp_leaf(d, 0, nullptr, nullptr, b.data());
+        spm_decomp_destroy(d);
+        for (size_t e = 0; e < b.size(); ++e) leaf->mat[e] = (S)b[e];
+        std::vector<int64_t> ai;
+        const int opt = this->m_Settings.algorithm == REGEFFICIENTCODEGENERATION ? SPM_CG_REGISTER_EFFICIENT : 0;
+        if (n >= 5 && superman_amd::integral(leaf, ai)) {
+            const int st = spm_sparse_codegen(ai.data(), n, opt, this->m_Settings.deviceID, nullptr, &last);
+            if (st != SPM_ERANGE) {
+                delete leaf;
+                superman_amd::check(st, "spm_sparse_codegen");
+                return last.re;
+            }
+        }
+        DecomposedGPU<C, S> generic(leaf, this->m_Settings);
         const double v = generic.computePermanent();
         last = generic.last;
         return v;

@@ -162,6 +162,44 @@ int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk, uint64_t* 
 /* Name of the permanent kernel a plan launches (for profiler lookups). */
 const char* spm_plan_kernel_name(const spm_plan* plan);
 
+/* ---- structural preprocessing (sparse inputs) ---------------------------------
+ * The reference's pipeline in front of its kernels, with the same
+ * semantics: fine Dulmage-Mendelsohn pruning of entries in no perfect
+ * matching (IO/IO.h:477-508 with IO/dm.cpp:372-501), connected components
+ * whose permanents multiply (IO/dm.h:43-107, main.cpp:73-111), and
+ * DecomposePerman's compressions of blocks larger than 30 -- degree-1
+ * expansion, degree-2 column merging, degree-3/4 splitting into two
+ * matrices whose permanents add (CommonFiles/DecomposePerman.h:119-557).
+ * The permanent is unchanged: perm(a) = prod over components of the sum of
+ * the component's leaves' permanents.  Host only. */
+#define SPM_DECOMP_DM          1  /* prune entries outside all perfect matchings */
+#define SPM_DECOMP_COMPONENTS  2  /* split into connected components              */
+#define SPM_DECOMP_COMPRESS    4  /* DecomposePerman compressions above order 30  */
+#define SPM_DECOMP_CODEGEN     8  /* code-generation mode: no component split and
+                                     degree-1/2 compressions only (one leaf), as
+                                     the reference does for its generated kernels */
+#define SPM_DECOMP_SMALL      16 /* compress above order 8 instead of 30 (lets tests
+                                     exercise every compression on small inputs) */
+#define SPM_DECOMP_ALL_FLAGS  31
+#define SPM_DECOMP_DEFAULT     7  /* DM | COMPONENTS | COMPRESS */
+typedef struct spm_decomp spm_decomp;
+int spm_decompose(const double* a, int n, int options, spm_decomp** out);
+/* Components, leaves, average leaf order and nonzeros (the reference's
+ * "Submatrix count / Average N / Average NNZ" line, main.cpp:115-118), and
+ * whether the permanent is structurally zero. */
+int spm_decomp_stats(const spm_decomp* d, int* components, int* leaves, double* avg_n, double* avg_nnz,
+                     int* structurally_zero);
+/* Leaf `leaf`: its component, order and (if out != NULL) its n x n entries. */
+int spm_decomp_leaf(const spm_decomp* d, int leaf, int* component, int* n, double* out);
+int spm_decomp_destroy(spm_decomp* d);
+/* The permanent through the decomposition: leaves of integer matrices run
+ * exactly (spm_int_exact) and combine in 128-bit integers (is_exact = 1
+ * while that range holds), others through spm_dense_f64 with `precision`.
+ * Replaces the DecomposePerman<...>::computePermanentRecursively drivers
+ * (CommonFiles/DecomposePerman.h:76-117) around the GPU algorithms. */
+int spm_permanent_decomposed(const double* a, int n, int options, int precision, int device, void* stream,
+                             spm_result* out);
+
 /* ---- batched and served permanents -----------------------------------------
  * No single counterpart in the reference, which computes one matrix per
  * driver call (main.cpp:69-115 calls the selected Algorithm once per

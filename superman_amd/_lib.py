@@ -34,6 +34,13 @@ SPM_INT_WALK_AUTO = 0
 SPM_INT_WALK_INT32 = 1
 SPM_INT_WALK_FP64 = 2
 
+SPM_DECOMP_DM = 1
+SPM_DECOMP_COMPONENTS = 2
+SPM_DECOMP_COMPRESS = 4
+SPM_DECOMP_CODEGEN = 8
+SPM_DECOMP_SMALL = 16
+SPM_DECOMP_DEFAULT = 7
+
 SPM_KIND_REAL = 1
 SPM_KIND_COMPLEX = 2
 SPM_KIND_INT = 3
@@ -133,6 +140,13 @@ PROTOTYPES = {
     "spm_gray_geometry": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_int)]),
+    "spm_decompose": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "spm_decomp_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int)]),
+    "spm_decomp_leaf": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _D]),
+    "spm_decomp_destroy": (ctypes.c_int, [_VP]),
+    "spm_permanent_decomposed": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_batch_config_default": (None, [ctypes.POINTER(SpmBatchConfig)]),
     "spm_batch_create": (ctypes.c_int, [ctypes.POINTER(SpmBatchConfig), ctypes.POINTER(_VP)]),
     "spm_batch_destroy": (ctypes.c_int, [_VP]),

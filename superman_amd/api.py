@@ -278,3 +278,48 @@ class CodegenPlan(DensePlan):
         sec, cached, h = ctypes.c_double(), ctypes.c_int(), ctypes.c_uint64()
         check(lib().spm_plan_codegen_info(self._h, ctypes.byref(sec), ctypes.byref(cached), ctypes.byref(h)))
         return {"compile_seconds": sec.value, "cached": bool(cached.value), "hash": h.value}
+
+
+@dataclass
+class Decomposition:
+    components: int
+    structurally_zero: bool
+    avg_n: float
+    avg_nnz: float
+    leaves: list  # [(component, ndarray)]
+
+
+def decompose(a, options: int = _lib.SPM_DECOMP_DEFAULT) -> Decomposition:
+    """The reference's preprocessing (DM pruning, components, compressions)
+    on the host: perm(a) = prod over components of sum of leaf permanents."""
+    a = _square(a, np.float64)
+    h = ctypes.c_void_p()
+    check(lib().spm_decompose(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0], options,
+                              ctypes.byref(h)), "spm_decompose")
+    try:
+        comps, nl, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        an, az = ctypes.c_double(), ctypes.c_double()
+        check(lib().spm_decomp_stats(h, ctypes.byref(comps), ctypes.byref(nl), ctypes.byref(an), ctypes.byref(az),
+                                     ctypes.byref(z)))
+        leaves = []
+        for k in range(nl.value):
+            c, n = ctypes.c_int(), ctypes.c_int()
+            check(lib().spm_decomp_leaf(h, k, ctypes.byref(c), ctypes.byref(n), None))
+            m = np.zeros((n.value, n.value))
+            check(lib().spm_decomp_leaf(h, k, None, None, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+            leaves.append((c.value, m))
+        return Decomposition(comps.value, bool(z.value), an.value, az.value, leaves)
+    finally:
+        lib().spm_decomp_destroy(h)
+
+
+def permanent_decomposed(a, precision="kahan", options: int = _lib.SPM_DECOMP_DEFAULT, device: int = 0,
+                         stream: int | None = None) -> Permanent:
+    """perm(a) through the decomposition, each leaf on the GPU (exact for
+    integer matrices while 128 bits hold the result)."""
+    a = _square(a, np.float64)
+    r = SpmResult()
+    check(lib().spm_permanent_decomposed(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0], options,
+                                         _prec(precision), device, stream, ctypes.byref(r)),
+          "spm_permanent_decomposed")
+    return _to_perm(r, False)

@@ -41,6 +41,7 @@ SOURCES = [
     ("capi.cpp", []),
     ("prep.cpp", []),
     ("batch_engine.cpp", []),
+    ("decompose.cpp", []),
     ("codegen.cpp", []),
 ]
 LINK_LIBS = ["-lhiprtc", "-lpthread"]

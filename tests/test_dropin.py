@@ -38,4 +38,4 @@ def test_unsupported_algorithm_raises():
 
 def test_no_device_is_an_error_not_a_fallback():
     rc, out = dropin.run("run", W.dense_uniform(8, 3), algorithm=XREGISTERMSHARED)
-    assert rc == 3 and out[-1].startswith("error spm_dense_f64")
+    assert rc == 3 and out[-1].startswith("error spm_permanent_decomposed")
