@@ -6,9 +6,9 @@ it.  Importing the package does not load the library; the first call does,
 and raises if the library has not been built (`python -m superman_amd.build`).
 """
 from .api import (CodegenPlan, DensePlan, Permanent, apply_permutations, codegen_compile_only,  # noqa: F401
-                  codegen_source, dense_c128, dense_f64, device_count, device_info, int_exact, is_integral,
-                  permanent, sparse_codegen, synchronize, ut_order)
+                  codegen_source, decompose, dense_c128, dense_f64, device_count, device_info, int_exact,
+                  is_integral, permanent, permanent_decomposed, sparse_codegen, synchronize, ut_order)
 
 __all__ = ["CodegenPlan", "DensePlan", "Permanent", "apply_permutations", "codegen_compile_only", "codegen_source",
-           "dense_c128", "dense_f64", "device_count", "device_info", "int_exact", "is_integral", "permanent",
-           "sparse_codegen", "synchronize", "ut_order"]
+           "decompose", "dense_c128", "dense_f64", "device_count", "device_info", "int_exact", "is_integral",
+           "permanent", "permanent_decomposed", "sparse_codegen", "synchronize", "ut_order"]
