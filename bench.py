@@ -61,10 +61,32 @@ def cpu_baseline(a: np.ndarray, target_s: float = 12.0) -> dict:
     O.dense_ref_sample(a, cores, count)
     dt = time.perf_counter() - t0
     rate = count / dt  # subsets per second
-    return {"value": rate / float(1 << (n - 1)), "unit": "perm/s", "cores": cores, "kind": "port",
-            "sample": f"reference dpNaivePerman walk (oracle/ryser_oracle.c orc_dense_ref_sample) over "
-                      f"Gray indices [1, 1+{count}) of the same n={n} matrix on {cores} threads, {dt:.1f} s; "
-                      f"{rate:.3e} subsets/s scaled to 2^{n - 1} subsets per permanent"}
+    out = {"value": rate / float(1 << (n - 1)), "unit": "perm/s", "cores": cores, "kind": "port",
+           "sample": f"reference dpNaivePerman walk (oracle/ryser_oracle.c orc_dense_ref_sample) over "
+                     f"Gray indices [1, 1+{count}) of the same n={n} matrix on {cores} threads, {dt:.1f} s; "
+                     f"{rate:.3e} subsets/s scaled to 2^{n - 1} subsets per permanent"}
+    out["reference_check"] = reference_check(cores)
+    return out
+
+
+def reference_check(cores: int, n: int = 30) -> dict | None:
+    """The reference's own CPU binary (oracle/_ref, built from its sources) and
+    the port on one whole n=30 permanent of the same generator, same threads:
+    the port's speed stands in for the reference's on the n=40 sample."""
+    from oracle import oracle as O
+    from oracle import refbin
+    if not refbin.available():
+        return None
+    from superman_amd import workloads as W
+    a = W.dense_normal(n, n)
+    t0 = time.perf_counter()
+    v_port = O.dense_ref(a, cores)
+    t_port = time.perf_counter() - t0
+    v_ref, t_ref, _ = refbin.run(a, threads=cores)
+    sub = float(1 << (n - 1))
+    return {"n": n, "threads": cores, "reference_s": t_ref, "port_s": t_port,
+            "reference_subsets_per_s": sub / t_ref, "port_subsets_per_s": sub / t_port,
+            "values_rel_diff": abs(v_port - v_ref) / abs(v_ref)}
 
 
 def measured_traffic(kernel: str):
