@@ -85,9 +85,31 @@ def build(jobs: int = 8, verbose: bool = False, defines=(), out: str | None = No
     if out.returncode != 0:
         raise RuntimeError(f"link failed\n{out.stdout}\n{out.stderr}")
     os.replace(tmp, lib_path)
+    if out_is_default(lib_path):
+        build_cli(verbose)
     if verbose:
         print(f"built {lib_path} in {time.time() - t0:.1f}s", flush=True)
     return lib_path
+
+
+CLI = os.path.join(PKG, "superman_gpu")
+
+
+def out_is_default(lib_path: str) -> bool:
+    return os.path.abspath(lib_path) == os.path.abspath(LIB)
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The command-line driver (csrc/cli_main.cpp): host C++ only, linked to
+    the in-tree library with an $ORIGIN run path."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "cli_main.cpp"), "-L" + PKG, "-lsuperman_amd", "-Wl,-rpath,$ORIGIN", "-o", CLI]
+    if verbose:
+        print(" ".join(shlex.quote(c) for c in cmd), flush=True)
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError(f"CLI build failed\n{out.stdout}\n{out.stderr}")
+    return CLI
 
 
 def main(argv=None):
