@@ -48,8 +48,8 @@ def run(a=None, *args, pattern=False, path=None):
 
 
 def value(stdout):
-    m = re.search(r"Permanent: (\S+) - Computed in", stdout)
+    m = re.search(r"Permanent: \(([^,]+),([^)]+)\) - Computed in", stdout)
     if m:
-        return float(m.group(1))
-    m = re.search(r"Permanent: \(([^,]+),([^)]+)\)", stdout)
-    return complex(float(m.group(1)), float(m.group(2)))
+        return complex(float(m.group(1)), float(m.group(2)))
+    m = re.search(r"Permanent: (\S+) - Computed in", stdout)
+    return float(m.group(1))
