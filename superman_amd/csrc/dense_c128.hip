@@ -17,16 +17,17 @@ namespace spm {
 namespace cplx {
 
 // Minimum waves per SIMD asked of the register allocator, per order: 3 for
-// 29..36, where 168 VGPRs keep the 2-step walk free of spills (the one-off
+// 29..35, where 168 VGPRs keep the 2-step walk free of spills (the one-off
 // chunk-start code may spill a few values) and the compiler's own choice
-// (up to 200 VGPRs) leaves 2 -- measured 15% faster at n = 32; 1 = the
-// compiler's choice (forcing 4 at n = 24 measured slower).
+// (up to 200 VGPRs) leaves 2 -- measured 15% faster at n = 32; at 36 the
+// walk itself would spill; 1 = the compiler's choice (forcing 4 at n = 24
+// measured slower).
 // SPM_C128_WAVES_PER_EU (sweeps only) overrides it for every order.
 #ifndef SPM_C128_WAVES_PER_EU
 #define SPM_C128_WAVES_PER_EU 0
 #endif
 template <int N>
-constexpr int kC128Waves = SPM_C128_WAVES_PER_EU > 0 ? SPM_C128_WAVES_PER_EU : (N >= 29 && N <= 36) ? 3 : 1;
+constexpr int kC128Waves = SPM_C128_WAVES_PER_EU > 0 ? SPM_C128_WAVES_PER_EU : (N >= 29 && N <= 35) ? 3 : 1;
 
 template <int N, bool COMP>
 __global__ void __launch_bounds__(kDenseBlock) __attribute__((amdgpu_waves_per_eu(kC128Waves<N>)))

@@ -12,11 +12,11 @@
 //
 // x is complex: re/im interleaved in 2N doubles (4N VGPRs).  A column is 2N
 // doubles (4N dwords: 128 SGPRs at N = 32), more than the SGPR file can hold
-// next to the loop state, so each flip streams its column in 8-double
-// chunks with one chunk of lookahead, and the first chunk of the next column
-// is prefetched across the current step's products.  Every lambda of the
-// walk is force-inlined: an outlined one takes x by reference and sends it to
-// scratch memory.
+// next to the loop state, so each flip takes its column in parts of at most
+// 32 doubles, one part per phase of rows, and loads the next part while
+// the product runs over the rows the previous part updated (gray_step).
+// Every lambda of the walk is force-inlined: an outlined one takes x by
+// reference and sends it to scratch memory.
 //
 // Per subset: 2N adds (the flip), 4(N-2) mul/FMA for the product of N-1
 // rows (complex product as mul + FMA per part), 4 FMAs to accumulate the
@@ -73,7 +73,7 @@ struct Chunk {
     double v[kChunk];
 };
 
-// chunk k of a column (doubles [16k, 16k+16) clipped to 2N)
+// chunk K of a column (doubles [8K, 8K+8) clipped to 2N)
 template <int N, int K>
 __device__ __forceinline__ void load_chunk(Chunk& c, const double* col)
 {
@@ -81,30 +81,6 @@ __device__ __forceinline__ void load_chunk(Chunk& c, const double* col)
 #pragma unroll
     for (int i = 0; i < kChunk; ++i)
         if (K * kChunk + i < 2 * N) c.v[i] = q[i];
-}
-
-// x += s * column (FMA = per-lane direction) or x += column, with chunk 0
-// already loaded in c0.
-template <int N, bool FMA>
-__device__ __forceinline__ void apply_column(double (&x)[2 * N], const Chunk& c0, const double* col, double s)
-{
-    Chunk cur = c0;
-    static_for<0, kChunks<N>>([&](auto KK) __attribute__((always_inline)) {
-        constexpr int k = decltype(KK)::value;
-        Chunk nxt;
-        if constexpr (k + 1 < kChunks<N>) load_chunk<N, k + 1>(nxt, col);
-        fence();
-#pragma unroll
-        for (int i = 0; i < kChunk; ++i) {
-            constexpr int base = k * kChunk;
-            if (base + i < 2 * N) {
-                if constexpr (FMA) x[base + i] = __builtin_fma(s, cur.v[i], x[base + i]);
-                else x[base + i] += cur.v[i];
-            }
-        }
-        fence();
-        if constexpr (k + 1 < kChunks<N>) cur = nxt;
-    });
 }
 
 // x = column (MODE 0), x += column (1) or x = fma(s, column, x) (2), the
@@ -129,6 +105,35 @@ __device__ __forceinline__ void stream_column(double (&x)[2 * N], const double* 
     });
 }
 
+// The column of a flip is consumed in kPhases<N> parts of at most
+// kPartMax doubles (2 kPhaseRows<N> doubles = the rows of one phase).  Part
+// p+1 is loaded while the product runs over the rows that part p just
+// updated, and the next flip's part 0 while the product finishes: every
+// wait for a scalar load follows a product segment, so no load is waited
+// on right after it is issued (scalar loads complete out of order, so each
+// wait is for all of them).
+constexpr int kPartMax = 32;  // doubles (64 SGPRs)
+template <int N>
+constexpr int kPhases = (2 * N + kPartMax - 1) / kPartMax;
+template <int N>
+constexpr int kPhaseRows = (N + kPhases<N> - 1) / kPhases<N>;
+
+template <int N>
+struct Part {
+    double v[2 * kPhaseRows<N>];
+};
+
+// Part P of a column: doubles [2 P R, 2 min(N, (P+1) R)), R = kPhaseRows<N>.
+template <int N, int P>
+__device__ __forceinline__ void load_part(Part<N>& c, const double* col)
+{
+    constexpr int b = 2 * P * kPhaseRows<N>;
+    const const_f64* __restrict__ q = step_ptr(col + b);
+#pragma unroll
+    for (int i = 0; i < 2 * kPhaseRows<N>; ++i)
+        if (b + i < 2 * N) c.v[i] = q[i];
+}
+
 // (ar, ai) *= (br, bi): mul + FMA per part.
 __device__ __forceinline__ void cmul(double& ar, double& ai, double br, double bi)
 {
@@ -140,10 +145,38 @@ __device__ __forceinline__ void cmul(double& ar, double& ai, double br, double b
     ai = i;
 }
 
-// Product of rows 0..N-2 (four chains), then blk += sign * p * x[N-1].
+// Product of rows 0..N-2 in four chains (one for N <= 5), advanced over the
+// row range [R0, R1) so that it can be split between column parts.
+struct Prod {
+    double r[4], i[4];
+};
+
+template <int N, int R0, int R1>
+__device__ __forceinline__ void prod_rows(const double (&x)[2 * N], Prod& p)
+{
+    constexpr int H = N - 1;
+    static_for<R0, (R1 < H ? R1 : H)>([&](auto RR) __attribute__((always_inline)) {
+        constexpr int r = decltype(RR)::value;
+        if constexpr (H <= 4) {
+            if constexpr (r == 0) {
+                p.r[0] = x[0];
+                p.i[0] = x[1];
+            } else {
+                cmul(p.r[0], p.i[0], x[2 * r], x[2 * r + 1]);
+            }
+        } else if constexpr (r < 4) {
+            p.r[r] = x[2 * r];
+            p.i[r] = x[2 * r + 1];
+        } else {
+            cmul(p.r[r & 3], p.i[r & 3], x[2 * r], x[2 * r + 1]);
+        }
+    });
+}
+
+// blk += sign * (product of rows 0..N-2) * x[N-1].
 template <int N, bool COMP>
-__device__ __forceinline__ void add_term(const double (&x)[2 * N], bool negative, double& br, double& bi,
-                                         double& babs)
+__device__ __forceinline__ void finish_term(const double (&x)[2 * N], Prod& p, bool negative, double& br,
+                                            double& bi, double& babs)
 {
     constexpr int H = N - 1;
     double pr, pi;
@@ -151,25 +184,14 @@ __device__ __forceinline__ void add_term(const double (&x)[2 * N], bool negative
         pr = 1.0;
         pi = 0.0;
     } else if constexpr (H <= 4) {
-        pr = x[0];
-        pi = x[1];
-#pragma unroll
-        for (int r = 1; r < H; ++r) cmul(pr, pi, x[2 * r], x[2 * r + 1]);
+        pr = p.r[0];
+        pi = p.i[0];
     } else {
-        double c0r = x[0], c0i = x[1], c1r = x[2], c1i = x[3];
-        double c2r = x[4], c2i = x[5], c3r = x[6], c3i = x[7];
-#pragma unroll
-        for (int r = 4; r < H; ++r) {
-            if ((r & 3) == 0) cmul(c0r, c0i, x[2 * r], x[2 * r + 1]);
-            else if ((r & 3) == 1) cmul(c1r, c1i, x[2 * r], x[2 * r + 1]);
-            else if ((r & 3) == 2) cmul(c2r, c2i, x[2 * r], x[2 * r + 1]);
-            else cmul(c3r, c3i, x[2 * r], x[2 * r + 1]);
-        }
-        cmul(c0r, c0i, c1r, c1i);
-        cmul(c2r, c2i, c3r, c3i);
-        cmul(c0r, c0i, c2r, c2i);
-        pr = c0r;
-        pi = c0i;
+        cmul(p.r[0], p.i[0], p.r[1], p.i[1]);
+        cmul(p.r[2], p.i[2], p.r[3], p.i[3]);
+        cmul(p.r[0], p.i[0], p.r[2], p.i[2]);
+        pr = p.r[0];
+        pi = p.i[0];
     }
     const double lr = x[2 * N - 2], li = x[2 * N - 1];
     const double sr = negative ? -pr : pr, si = negative ? -pi : pi;
@@ -183,6 +205,43 @@ __device__ __forceinline__ void add_term(const double (&x)[2 * N], bool negative
         const double la = __builtin_fabs(lr) + __builtin_fabs(li);
         babs = __builtin_fma(pa, la, babs);
     }
+}
+
+template <int N, bool COMP>
+__device__ __forceinline__ void add_term(const double (&x)[2 * N], bool negative, double& br, double& bi,
+                                         double& babs)
+{
+    Prod p;
+    prod_rows<N, 0, N>(x, p);
+    finish_term<N, COMP>(x, p, negative, br, bi, babs);
+}
+
+// One Gray step: x += column `col` (x = fma(s, column, x) when FMA: the
+// per-lane direction of the special flip), whose part 0 is in `buf`, then
+// the term of the new subset.  Leaves part 0 of `next` loading into `buf`.
+template <int N, bool COMP, bool FMA>
+__device__ __forceinline__ void gray_step(double (&x)[2 * N], Part<N>& buf, const double* col, double s,
+                                          const double* next, bool negative, double& br, double& bi,
+                                          double& babs)
+{
+    Prod p;
+    static_for<0, kPhases<N>>([&](auto PP) __attribute__((always_inline)) {
+        constexpr int ph = decltype(PP)::value;
+        constexpr int r0 = ph * kPhaseRows<N>;
+        constexpr int r1 = (r0 + kPhaseRows<N> < N) ? r0 + kPhaseRows<N> : N;
+#pragma unroll
+        for (int i = 0; i < 2 * (r1 - r0); ++i) {
+            if constexpr (FMA) x[2 * r0 + i] = __builtin_fma(s, buf.v[i], x[2 * r0 + i]);
+            else x[2 * r0 + i] += buf.v[i];
+        }
+        fence();
+        if constexpr (ph + 1 < kPhases<N>) load_part<N, ph + 1>(buf, col);
+        else load_part<N, 0>(buf, next);
+        fence();
+        prod_rows<N, r0, r1>(x, p);
+        if constexpr (ph + 1 == kPhases<N>) finish_term<N, COMP>(x, p, negative, br, bi, babs);
+        fence();
+    });
 }
 
 // One lane's chunk: the 2^L Gray steps starting at k = h 2^L, folded into the
@@ -223,10 +282,9 @@ __device__ __forceinline__ void walk_c128(const double* __restrict__ M,   // [2]
         add_term<N, COMP>(x, false, br, bi, babs);
     } else if constexpr (N == 2) {  // L = 1: steps k = 0, 1; column 0 flips by lane
         add_term<N, COMP>(x, false, br, bi, babs);
-        Chunk c;
-        load_chunk<N, 0>(c, M);
-        apply_column<N, true>(x, c, M, lane_sign);
-        add_term<N, COMP>(x, true, br, bi, babs);
+        Part<N> buf;
+        load_part<N, 0>(buf, M);
+        gray_step<N, COMP, true>(x, buf, M, lane_sign, M, true, br, bi, babs);
     } else {
         // 2-step blocks as in dense_f64 (L >= 2): k = 2m flips column
         // ctz(m)+1, k = 2m+1 flips column 0; block nblk/2 (column L-1)
@@ -237,29 +295,17 @@ __device__ __forceinline__ void walk_c128(const double* __restrict__ M,   // [2]
             const int tz = __builtin_ctz(m);
             return (((m >> (tz + 1)) & 1u) ? Mneg : M) + (size_t)(tz + 1) * C;
         };
-        Chunk ca, cb;
-        const double* pa = a_col(1);
+        Part<N> buf;
         add_term<N, COMP>(x, false, br, bi, babs);  // k = 0
-        load_chunk<N, 0>(cb, M);
-        apply_column<N, false>(x, cb, M, 1.0);      // k = 1: +column 0
-        load_chunk<N, 0>(ca, pa);
+        load_part<N, 0>(buf, M);
+        const double* pa = a_col(1);
         fence();
-        add_term<N, COMP>(x, true, br, bi, babs);
-        fence();
+        gray_step<N, COMP, false>(x, buf, M, 1.0, pa, true, br, bi, babs);  // k = 1: +column 0
         auto block = [&](uint32_t m, auto SPECIAL) __attribute__((always_inline)) {
-            if constexpr (decltype(SPECIAL)::value) apply_column<N, true>(x, ca, pa, lane_sign);
-            else apply_column<N, false>(x, ca, pa, 1.0);
             const double* pb = (m & 1u) ? Mneg : M;
-            load_chunk<N, 0>(cb, pb);
-            fence();
-            add_term<N, COMP>(x, false, br, bi, babs);
-            fence();
-            apply_column<N, false>(x, cb, pb, 1.0);
+            gray_step<N, COMP, decltype(SPECIAL)::value>(x, buf, pa, lane_sign, pb, false, br, bi, babs);
             pa = a_col(m + 1);  // m+1 = nblk: column L <= N-1, a valid address
-            load_chunk<N, 0>(ca, pa);
-            fence();
-            add_term<N, COMP>(x, true, br, bi, babs);
-            fence();
+            gray_step<N, COMP, false>(x, buf, pb, 1.0, pa, true, br, bi, babs);
         };
         constexpr uint32_t G = kFoldSteps / 2;
         auto run = [&](uint32_t m, uint32_t end) __attribute__((always_inline)) {
