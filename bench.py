@@ -114,11 +114,11 @@ def serving(count: int, seed: int = 2026, load: float = 0.8) -> dict:
         tuning, _ = eng.autotune(reqs[:1000], reps=1)
         rc, res_sat, _, sat = eng.replay(reqs, np.zeros(count))  # capacity: everything queued at t = 0
         if rc != 0:
-            raise RuntimeError(f"batched capacity run failed: status {rc}")
+            raise RuntimeError(f"batched capacity run failed: status {rc} ({eng.last_error()})")
         rate = load * sat.perms_per_s
         rc, res, lat, st = eng.replay(reqs, unit_arrivals / rate)
         if rc != 0:
-            raise RuntimeError(f"batched Poisson run failed: status {rc}")
+            raise RuntimeError(f"batched Poisson run failed: status {rc} ({eng.last_error()})")
     batched = [to_permanent(r, q[0]) for r, q in zip(res, reqs)]
 
     # plain path: one C-ABI call per matrix, back to back (closed loop)
@@ -147,6 +147,7 @@ def serving(count: int, seed: int = 2026, load: float = 0.8) -> dict:
         "perms_per_s": st.perms_per_s, "offered_rate": rate, "load": load,
         "p50_ms": st.p50_s * 1e3, "p99_ms": st.p99_s * 1e3, "max_ms": st.max_s * 1e3,
         "batches": st.batches, "mean_batch": st.mean_batch, "device_busy": st.device_busy,
+        "late_publish": sat.late_publish + st.late_publish, "retried": sat.retried + st.retried,
         "capacity_perms_per_s": sat.perms_per_s, "capacity_batches": sat.batches,
         "plain_perms_per_s": plain_rate, "plain_call_p50_ms": float(pl[len(pl) // 2] * 1e3),
         "plain_call_p99_ms": float(pl[int(0.99 * (len(pl) - 1))] * 1e3),
@@ -182,6 +183,17 @@ def secondary() -> dict:
                                       "launches": d.launches,
                                       "pipeline": "DM pruning + components + DecomposePerman compressions, "
                                                   "leaves through the exact integer kernels"}
+    u = W.dense_uniform(40, 40)
+    with S.DensePlan(u, "kahan") as p:
+        r64 = p.run()
+    with S.DensePlan(u, "f32_product") as p:
+        rf = p.run()
+    out["dense_uniform_n40_f32_product"] = {
+        "kernel_ms": rf.kernel_ms, "value": rf.value, "err_est": rf.err_est,
+        "rel_err_est": rf.err_est / abs(rf.value), "fp64_kernel_ms": r64.kernel_ms, "fp64_value": r64.value,
+        "fp64_err_est": r64.err_est, "rel_diff_vs_fp64": abs(rf.value - r64.value) / abs(r64.value),
+        "variant": "rows scaled by powers of two, FP64 row sums, FP32 products of n-1 factors, compensated FP64 "
+                   "accumulation; nonnegative uniform[0,1) n=40, numpy default_rng(40)"}
     z = W.haar_block(32, 32)
     with S.DensePlan(z, "kahan") as p:
         p.run()

@@ -42,6 +42,11 @@ extern "C" {
 #define SPM_PREC_PLAIN        0   /* reference "dd": plain FP64 sums          */
 #define SPM_PREC_COMPENSATED  1   /* reference "kahan" (its default, IO.h:178):
                                      TwoSum-compensated double-double sums   */
+#define SPM_PREC_F32_PRODUCT  2   /* real, nonnegative entries only (else
+                                     SPM_EINVAL): rows scaled by powers of two,
+                                     FP64 row sums, FP32 products, compensated
+                                     FP64 accumulation; err_est bounds the FP32
+                                     product error (no reference counterpart) */
 
 /* Extended result.  `re` (and `im` for complex) is the value the reference
  * returns; hi/lo carry the double-double sum; `abs_sum` = 2 * sum |term|
@@ -266,6 +271,11 @@ typedef struct spm_serve_stats {
     double p50_s, p99_s, mean_s, max_s;  /* latency: arrival -> result on host      */
     double mean_batch;      /* matrices per batch                                   */
     double device_busy;     /* fraction of wall_s with a batch in flight            */
+    uint64_t late_publish;  /* results whose flag reached host memory after their
+                               batch's completion was seen (waited for, not lost) */
+    uint64_t retried;       /* requests run again after their result failed a check
+                               (stale item records, integer sum not a multiple of
+                               2^(n-1)); spm_batch_last_error names the check */
 } spm_serve_stats;
 
 typedef struct spm_batch spm_batch;
@@ -301,6 +311,10 @@ int spm_serve_replay(spm_batch* b, int count, const spm_request* reqs, const dou
 /* The next `launches` batch launches fail on the host without touching the
  * device; their requests complete with SPM_EFAULTINJ. */
 int spm_batch_inject_fault(spm_batch* b, int launches);
+/* The next `results` results handed out count as failing their checks (no
+ * reference counterpart; for tests of the re-run path): each such request
+ * runs again, up to twice, then completes with SPM_EHIP. */
+int spm_batch_inject_bad_results(spm_batch* b, int results);
 /* Which HIP call failed last (empty if none); valid until the next call on
  * the same thread. */
 const char* spm_batch_last_error(spm_batch* b);

@@ -29,6 +29,7 @@ SPM_CG_REGISTER_EFFICIENT = 1
 
 SPM_PREC_PLAIN = 0
 SPM_PREC_COMPENSATED = 1
+SPM_PREC_F32_PRODUCT = 2
 
 SPM_INT_WALK_AUTO = 0
 SPM_INT_WALK_INT32 = 1
@@ -89,7 +90,8 @@ class SpmServeStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
                 ("wall_s", ctypes.c_double), ("perms_per_s", ctypes.c_double), ("p50_s", ctypes.c_double),
                 ("p99_s", ctypes.c_double), ("mean_s", ctypes.c_double), ("max_s", ctypes.c_double),
-                ("mean_batch", ctypes.c_double), ("device_busy", ctypes.c_double)]
+                ("mean_batch", ctypes.c_double), ("device_busy", ctypes.c_double), ("late_publish", ctypes.c_uint64),
+                ("retried", ctypes.c_uint64)]
 
 
 class SpmError(RuntimeError):
@@ -163,6 +165,7 @@ PROTOTYPES = {
     "spm_serve_replay": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(SpmRequest), _D, _R, _D,
                                         ctypes.POINTER(SpmServeStats)]),
     "spm_batch_inject_fault": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "spm_batch_inject_bad_results": (ctypes.c_int, [_VP, ctypes.c_int]),
     "spm_batch_last_error": (ctypes.c_char_p, [_VP]),
 }
 

@@ -19,7 +19,8 @@ from . import _lib
 from ._lib import SpmResult, check, lib
 
 PRECISIONS = {"dd": _lib.SPM_PREC_PLAIN, "plain": _lib.SPM_PREC_PLAIN,
-              "kahan": _lib.SPM_PREC_COMPENSATED, "compensated": _lib.SPM_PREC_COMPENSATED}
+              "kahan": _lib.SPM_PREC_COMPENSATED, "compensated": _lib.SPM_PREC_COMPENSATED,
+              "f32_product": _lib.SPM_PREC_F32_PRODUCT}
 
 
 @dataclass

@@ -66,6 +66,8 @@ class ServeStats:
     max_s: float
     mean_batch: float
     device_busy: float
+    late_publish: int = 0
+    retried: int = 0
 
 
 class _Requests:
@@ -213,6 +215,9 @@ class BatchEngine:
     def inject_fault(self, launches: int = 1) -> None:
         check(lib().spm_batch_inject_fault(self._h, launches))
 
+    def inject_bad_results(self, results: int = 1) -> None:
+        check(lib().spm_batch_inject_bad_results(self._h, results))
+
     def replay(self, reqs, arrival_s) -> tuple[int, list, np.ndarray, ServeStats]:
         """Replays an open-loop arrival schedule; (status, [SpmResult], latency_s, stats)."""
         rq = _Requests(reqs)
@@ -225,7 +230,8 @@ class BatchEngine:
         rc = lib().spm_serve_replay(self._h, rq.count, rq.arr, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                     out, lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(st))
         stats = ServeStats(int(st.completed), int(st.failed), int(st.batches), st.wall_s, st.perms_per_s, st.p50_s,
-                           st.p99_s, st.mean_s, st.max_s, st.mean_batch, st.device_busy)
+                           st.p99_s, st.mean_s, st.max_s, st.mean_batch, st.device_busy, int(st.late_publish),
+                           int(st.retried))
         return rc, [out[i] for i in range(rq.count)], lat, stats
 
 

@@ -27,6 +27,9 @@
 //   integer : T mod 2^128 as (lo, hi) 64-bit words (both integer walks:
 //             int32 for general entries, FP64 when every absolute row sum
 //             is below 2^(bit width of n), e.g. 0/1 matrices)
+// and in every record word 7 = batch_tag(seq, item or request index); in a
+// result record word 6 = the number of item records the reduction found
+// without this batch's tag (0 unless the records were not yet visible).
 #include "spm_batch.h"
 #include "walk_c128.h"
 #include "walk_f64.h"
@@ -150,14 +153,21 @@ __device__ __forceinline__ void wave_combine(int kind, double (&o)[8])
 
 // A request's item records summed in a fixed order (lane l takes items
 // l, l+64, ..., then the butterfly); every lane writes the same total.
-__device__ __forceinline__ void reduce_request(int kind, const double* __restrict__ recs, uint32_t items,
-                                               uint32_t lane, double* __restrict__ out)
+// Word kBatchTag of each item record must hold the item's tag for this
+// batch; records that do not are counted into word kBatchStale of the
+// result (0 when every record was this batch's), and the result's own tag
+// goes into its word kBatchTag.
+__device__ __forceinline__ void reduce_request(int kind, const double* __restrict__ recs, uint32_t item_base,
+                                               uint32_t items, uint32_t seq, uint32_t r, uint32_t lane,
+                                               double* __restrict__ out)
 {
     double o[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     unsigned __int128 t = 0;
+    uint32_t stale = 0;
     const bool is_int = kind == kBatchInt || kind == kBatchIntSmall;
     for (uint32_t j = lane; j < items; j += kWave) {
         const double* rec = recs + (size_t)j * kBatchRec;
+        stale += __double_as_longlong(rec[kBatchTag]) != (long long)batch_tag(seq, item_base + j) ? 1u : 0u;
         if (is_int) {
             t += ((unsigned __int128)(unsigned long long)__double_as_longlong(rec[1]) << 64) |
                  (unsigned long long)__double_as_longlong(rec[0]);
@@ -172,6 +182,10 @@ __device__ __forceinline__ void reduce_request(int kind, const double* __restric
         o[1] = __longlong_as_double((long long)(unsigned long long)(t >> 64));
     }
     wave_combine(kind, o);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) stale += __shfl_xor(stale, off, kWave);
+    o[kBatchStale] = (double)stale;
+    o[kBatchTag] = __longlong_as_double((long long)batch_tag(seq, r));
 #pragma unroll
     for (int c = 0; c < kBatchRec; ++c) out[c] = o[c];
 }
@@ -224,6 +238,7 @@ batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_r
 #pragma unroll
         for (int c = 0; c < 8; ++c) o[c] = idle ? 0.0 : o[c];
         wave_combine(kind, o);
+        o[kBatchTag] = __longlong_as_double((long long)batch_tag(seq, i));
         double* rec = item_rec + (size_t)i * kBatchRec;
 #pragma unroll
         for (int c = 0; c < kBatchRec; ++c) rec[c] = o[c];
@@ -235,7 +250,8 @@ batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_r
         const uint32_t prev = uniform(atomicAdd(done + r, lane == 0 ? 1u : 0u));
         if (prev + 1 == items) {
             __threadfence();
-            reduce_request(kind, item_rec + (size_t)uniform(d->item_base) * kBatchRec, items, lane,
+            const uint32_t base = uniform(d->item_base);
+            reduce_request(kind, item_rec + (size_t)base * kBatchRec, base, items, seq, r, lane,
                            result_host + (size_t)r * kBatchRec);
             // publish: the record reaches host memory before the flag does
             __threadfence_system();

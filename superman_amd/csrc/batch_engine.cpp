@@ -41,6 +41,13 @@ namespace {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Seconds a finished batch may wait for a result's flag to reach host memory.
+constexpr double kPublishWait = 1.0;
+// Re-runs of a request whose result failed its checks (item records of
+// another batch in its reduction, an integer sum that is not a multiple of
+// 2^(n-1)) before it completes with SPM_EHIP.
+constexpr int kMaxRetries = 2;
+
 // FP64-op units per Gray subset (work estimates for budgets and ordering),
 // by device walk.
 double ops_per_subset(int dev_kind, int n)
@@ -69,6 +76,7 @@ struct Req {
     double t_submit = 0, t_deadline = 0, t_launch = 0;
     bool cancelled = false;
     int index = -1;             // position in a run/replay, -1 for served requests
+    int retries = 0;            // re-runs after a result failed its checks
 };
 
 struct Slot {
@@ -112,6 +120,9 @@ struct spm_batch {
     uint64_t batches = 0;
     double busy_since = -1, busy_total = 0;
     std::string last_error;  // the last failed HIP call, for spm_batch_last_error
+    uint64_t late_publish = 0;  // results whose flag arrived after their batch's completion event
+    uint64_t retried = 0;       // requests run again after a result failed its checks
+    int bad_results = 0;        // results still to be failed on purpose (spm_batch_inject_bad_results)
     std::mutex mu;
 };
 
@@ -184,6 +195,17 @@ int prepare(const spm_request* q, Req* r)
     return SPM_OK;
 }
 
+// Puts a request whose result failed its checks back at the head of the
+// queue; false once its re-runs are used up.
+bool requeue(spm_batch* b, Req* r)
+{
+    if (r->retries >= kMaxRetries || r->cancelled) return false;
+    ++r->retries;
+    ++b->retried;
+    b->pending.push_front(r);
+    return true;
+}
+
 void complete(spm_batch* b, Req* r, int status, const double* rec, double t_done, float batch_ms)
 {
     spm_completion c;
@@ -195,11 +217,28 @@ void complete(spm_batch* b, Req* r, int status, const double* rec, double t_done
     c.t_done = t_done;
     spm_result& res = c.result;
     if (status == SPM_OK && r->cancelled) status = SPM_ECANCELLED;
-    if (status == SPM_OK && rec) {
+    if (status == SPM_OK && b->bad_results > 0) {
+        --b->bad_results;
+        b->last_error = "injected bad result";
+        if (requeue(b, r)) return;
+        status = SPM_EHIP;
+    } else if (status == SPM_OK && rec && rec[kBatchStale] != 0.0) {
+        // the reduction read item records that were not this batch's
+        status = SPM_EHIP;
+        b->last_error = "batch execution: " + std::to_string((long long)rec[kBatchStale]) + " of " +
+                        std::to_string(r->items) + " item records of a request (n = " + std::to_string(r->n) +
+                        ") were not this batch's when its last item finished";
+        if (requeue(b, r)) return;
+    } else if (status == SPM_OK && rec) {
         if (r->kind == SPM_KIND_INT) {
             uint64_t w[2];
             std::memcpy(w, rec, 16);
             status = int_fill_result(r->n, w[0], w[1], &res);
+            if (status != SPM_OK)
+                b->last_error = "batch execution: integer sum of a request (n = " + std::to_string(r->n) +
+                                ", L = " + std::to_string(r->L) + ", " + std::to_string(r->items) +
+                                " items) is not a multiple of 2^(n-1)";
+            if (status != SPM_OK && requeue(b, r)) return;
         } else {
             dense_fill_result(r->n, r->kind == SPM_KIND_COMPLEX, true, rec, &res);
         }
@@ -434,6 +473,17 @@ void fail_slot(spm_batch* b, Slot& s, int status, double now)
     s.busy = false;
 }
 
+// A request's result is on the host: its flag holds the slot's batch number
+// and its record carries its own tag (the record's words are stored before
+// the flag, but each read here re-checks both).
+bool published(const Slot& s, size_t j)
+{
+    if (__atomic_load_n(s.h_flag + j, __ATOMIC_ACQUIRE) != s.seq) return false;
+    uint64_t tag;
+    std::memcpy(&tag, s.h_res + j * kBatchRec + kBatchTag, sizeof tag);
+    return tag == batch_tag(s.seq, (uint32_t)j);
+}
+
 // Retires finished slots; returns how many were retired.
 int retire(spm_batch* b, double now)
 {
@@ -453,9 +503,9 @@ int retire(spm_batch* b, double now)
             ++got;
             continue;
         }
-        // results already published by the kernel
+        // results already published by the kernel (flag and record tag)
         for (size_t j = 0; j < s.reqs.size(); ++j) {
-            if (s.taken[j] || __atomic_load_n(s.h_flag + j, __ATOMIC_ACQUIRE) != s.seq) continue;
+            if (s.taken[j] || !published(s, j)) continue;
             s.taken[j] = 1;
             complete(b, s.reqs[j], SPM_OK, s.h_res + j * kBatchRec, now, -1.f);
             s.reqs[j] = nullptr;
@@ -471,10 +521,19 @@ int retire(spm_batch* b, double now)
             ++got;
             continue;
         }
-        // the batch is over: every flag must be set by now
+        // The batch's kernels are over, so every result has been stored; a
+        // store to host memory may still be on its way when the completion
+        // is seen, so each missing flag gets a bounded wait before it counts
+        // as an error.
         for (size_t j = 0; j < s.reqs.size(); ++j) {
             if (s.taken[j]) continue;
-            const bool ok = __atomic_load_n(s.h_flag + j, __ATOMIC_ACQUIRE) == s.seq;
+            bool ok = published(s, j);
+            if (!ok) {
+                const double give_up = now_seconds() + kPublishWait;
+                while (!(ok = published(s, j)) && now_seconds() < give_up) std::this_thread::yield();
+                if (ok) ++b->late_publish;
+                else b->last_error = "batch execution: a result was not published by the end of its batch";
+            }
             complete(b, s.reqs[j], ok ? SPM_OK : SPM_EHIP, ok ? s.h_res + j * kBatchRec : nullptr, now, -1.f);
         }
         s.reqs.clear();
@@ -549,6 +608,8 @@ int drive(spm_batch* b, int count, const spm_request* reqs, const double* arriva
     std::unordered_map<uint64_t, int> index;
     std::vector<double> lat(count, 0.0);
     const uint64_t batches0 = b->batches;
+    const uint64_t late0 = b->late_publish;
+    const uint64_t retried0 = b->retried;
     b->busy_total = 0;
     b->busy_since = -1;
     const double t0 = now_seconds();
@@ -612,6 +673,8 @@ int drive(spm_batch* b, int count, const spm_request* reqs, const double* arriva
         }
         stats->completed = ok.size();
         stats->batches = b->batches - batches0;
+        stats->late_publish = b->late_publish - late0;
+        stats->retried = b->retried - retried0;
         const double first = arrival_s && count > 0 ? t0 + arrival_s[0] : t0;
         stats->wall_s = t_last - first;
         stats->perms_per_s = stats->wall_s > 0 ? ok.size() / stats->wall_s : 0.0;
@@ -930,5 +993,13 @@ extern "C" int spm_batch_inject_fault(spm_batch* b, int launches)
     if (!b || launches < 0) return SPM_EINVAL;
     std::lock_guard<std::mutex> lk(b->mu);
     b->fault_launches = launches;
+    return SPM_OK;
+}
+
+extern "C" int spm_batch_inject_bad_results(spm_batch* b, int results)
+{
+    if (!b || results < 0) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    b->bad_results = results;
     return SPM_OK;
 }

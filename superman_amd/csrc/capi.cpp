@@ -140,6 +140,7 @@ struct spm_plan {
     double* d_M = nullptr;   // dense kinds; the integer kind stores int32 here
     double* d_x0 = nullptr;
     double log2_bound = 0;   // integer kind: log2 of the bound on |T|
+    int log2_scale = 0;      // FP32-product plans: perm = 2^log2_scale perm(scaled)
     double* d_partial = nullptr;
     double* d_out = nullptr;
     double* h_out = nullptr;
@@ -182,9 +183,11 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     const int maxn = kind == 1 ? kMaxN : kMaxComplexN;
     const int w = kind == 1 ? 1 : 2;  // doubles per entry
     if (!a || n < 1 || n > maxn) return SPM_EINVAL;
-    if (precision != SPM_PREC_PLAIN && precision != SPM_PREC_COMPENSATED) return SPM_EINVAL;
+    const bool f32p = precision == SPM_PREC_F32_PRODUCT;
+    if (precision != SPM_PREC_PLAIN && precision != SPM_PREC_COMPENSATED && !(kind == 1 && f32p))
+        return SPM_EINVAL;
     for (int e = 0; e < w * n * n; ++e)
-        if (!std::isfinite(a[e])) return SPM_EINVAL;
+        if (!std::isfinite(a[e]) || (f32p && a[e] < 0.0)) return SPM_EINVAL;
     int cus = 0;
     int st = device_cus(device, &cus);
     if (st != SPM_OK) return st;
@@ -197,7 +200,7 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     p->precision = precision;
     p->device = device;
     const bool comp = precision == SPM_PREC_COMPENSATED;
-    p->kern = kind == 1 ? dense_f64_kernel_ptr(n, comp) : dense_c128_kernel_ptr(n, comp);
+    p->kern = f32p ? dense_f32p_kernel_ptr(n) : kind == 1 ? dense_f64_kernel_ptr(n, comp) : dense_c128_kernel_ptr(n, comp);
     p->rec = kind == 1 ? 4 : 8;
     p->comps = kind == 1 ? 1 : 2;
     if (!p->kern) {
@@ -217,11 +220,13 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
     // Split very long runs (~> 4 s of VALU work at 64 lanes x 256 CUs x 2.4 GHz)
     // into several launches that accumulate into the same per-block records.
-    const double ops = kind == 1 ? 2.0 * n : 6.0 * n;
+    const double ops = f32p ? 3.0 * n : kind == 1 ? 2.0 * n : 6.0 * n;
     const double est_s = std::ldexp(1.0, n - 1) * ops / 3.9e13;
     p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
     char kn[96];
-    if (kind == 1)
+    if (f32p)
+        std::snprintf(kn, sizeof kn, "dense_f32p_kernel<%d, %d>", n, ub);
+    else if (kind == 1)
         std::snprintf(kn, sizeof kn, "dense_f64_kernel<%d, %d, %s>", n, ub, comp ? "true" : "false");
     else
         std::snprintf(kn, sizeof kn, "dense_c128_kernel<%d, %s>", n, comp ? "true" : "false");
@@ -230,7 +235,12 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     // Host preparation (dpSingleGPU.cuh:62-82; gpuComputeComplex.cu:212-241).
     const size_t col = (size_t)w * n;   // doubles per column
     const size_t plane = col * n;
-    std::vector<double> colmaj(2 * plane), x0(col);
+    std::vector<double> colmaj(2 * plane), x0(col), scaled;
+    if (f32p) {
+        scaled.resize((size_t)n * n);
+        p->log2_scale = f32prod_scale(a, n, scaled.data());
+        a = scaled.data();
+    }
     dense_prepare(a, n, w, colmaj.data(), x0.data());
     SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * plane));
     SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * col));
@@ -496,7 +506,8 @@ static int dense_run(spm_plan* p, hipStream_t st, spm_result* r)
 
     std::memset(r, 0, sizeof *r);
     const int n = p->n;
-    dense_fill_result(n, p->comps == 2, p->precision == SPM_PREC_COMPENSATED, p->h_out, r);
+    dense_fill_result(n, p->comps == 2, p->precision != SPM_PREC_PLAIN, p->h_out, r);
+    if (p->precision == SPM_PREC_F32_PRODUCT) f32prod_finish(n, p->log2_scale, r);
     r->status = SPM_OK;
     r->kernel_ms = ms;
     r->subsets = uint64_t(1) << (n - 1);

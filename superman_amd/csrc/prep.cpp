@@ -117,6 +117,47 @@ void dense_fill_result(int n, bool complex_kind, bool compensated, const double*
     r->n = n;
 }
 
+// FP32-product runs: every half row sum r_i/2 of the scaled matrix lies in
+// [2^c, 2^(c+1)), so |x_i| < 2^(c+1) and a product of n-1 factors stays
+// below 2^127 (FP32 range) when (c+1) n <= 127.
+int f32prod_scale_exponent(int n)
+{
+    return 127 / n - 1;
+}
+
+// scaled = D a with D = diag(2^-e_i), exact (powers of two) unless a scaled
+// entry falls below the FP64 normal range; returns sum e_i, perm(a) =
+// 2^(sum e_i) perm(scaled).  Zero rows stay zero (e_i = 0).
+int f32prod_scale(const double* a, int n, double* scaled)
+{
+    const int c = f32prod_scale_exponent(n);
+    int total = 0;
+    for (int i = 0; i < n; ++i) {
+        double r = 0.0;
+        for (int j = 0; j < n; ++j) r += a[(size_t)i * n + j];
+        int e = 0;
+        if (r > 0.0) e = std::ilogb(0.5 * r) - c;
+        for (int j = 0; j < n; ++j) scaled[(size_t)i * n + j] = std::ldexp(a[(size_t)i * n + j], -e);
+        total += e;
+    }
+    return total;
+}
+
+// Error bound and scale of an FP32-product result.  A term whose FP32
+// product falls below 2^-126 loses relative accuracy: its absolute error is
+// below 2^-149 |x_last| < 2^(c+1-149), over 2^(n-1) terms (times the final
+// factor 2).
+void f32prod_finish(int n, int log2_scale, spm_result* r)
+{
+    const int c = f32prod_scale_exponent(n);
+    r->err_est = f32prod_err_est(n, r->abs_sum) + std::ldexp(1.0, n + c + 1 - 149);
+    r->re = std::ldexp(r->re, log2_scale);
+    r->hi_re = std::ldexp(r->hi_re, log2_scale);
+    r->lo_re = std::ldexp(r->lo_re, log2_scale);
+    r->abs_sum = std::ldexp(r->abs_sum, log2_scale);
+    r->err_est = std::ldexp(r->err_est, log2_scale);
+}
+
 // T mod 2^128 -> perm = (-1)^(n-1) T / 2^(n-1).  T must be a multiple of
 // 2^(n-1); a violation means a corrupted run and is reported as a HIP error.
 int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r)

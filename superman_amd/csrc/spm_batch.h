@@ -12,6 +12,19 @@ constexpr int kBatchComplex = 2;    // compensated complex128
 constexpr int kBatchInt = 3;        // exact integer (mod 2^128), int32 walk
 constexpr int kBatchIntSmall = 4;   // exact integer, small row sums: FP64 walk
 constexpr int kBatchRec = 8;        // doubles per item / result record
+constexpr int kBatchStale = 6;      // result word: item records of another batch seen by the reduction
+constexpr int kBatchTag = 7;        // item / result word: batch_tag(seq, item or request index)
+
+// Tag of item or request `idx` in the batch numbered `seq`, kept in word
+// kBatchTag of its record so that a reader can tell this batch's record
+// from an older one at the same place.
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline uint64_t batch_tag(uint32_t seq, uint32_t idx)
+{
+    return ((uint64_t)seq << 32) | idx;
+}
 constexpr int kBatchAlign = 64;     // byte alignment of every arena region
 
 struct BatchHeader {                // 64 bytes at arena offset 0

@@ -40,6 +40,14 @@ inline double complex_err_est(int n, double abs_sum)
     return 4.0 * n * u * abs_sum;
 }
 
+// FP32-product runs: each term's first-order relative error is below
+// (2n-3) 2^-24 (dense_f32p_kernel), on top of the FP64 accumulation's.
+inline double f32prod_err_est(int n, double abs_sum)
+{
+    const double u32 = 5.9604644775390625e-08;  // 2^-24
+    return (2.0 * n * u32 + dense_err_est(n, 1.0)) * abs_sum;
+}
+
 // prep.cpp -- per-matrix host preparation and result conversion
 void dense_prepare(const double* a, int n, int w, double* planes, double* x0);
 int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound);
@@ -47,6 +55,9 @@ void int_planes(const int64_t* a, int n, int* planes);
 bool int_small_ok(const int64_t* a, int n);
 void int_small_planes(const int64_t* a, int n, double* planes);
 void dense_fill_result(int n, bool complex_kind, bool compensated, const double* rec, spm_result* r);
+int f32prod_scale_exponent(int n);
+int f32prod_scale(const double* a, int n, double* scaled);
+void f32prod_finish(int n, int log2_scale, spm_result* r);
 int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r);
 
 // codegen.cpp -- SpaRyser per-pattern kernels

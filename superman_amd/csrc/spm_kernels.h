@@ -16,6 +16,8 @@ constexpr int kDenseBlock = 256;   // threads per block of the dense kernels
 //          uint64_t task_begin, uint64_t task_end, uint64_t num_chunks, int L,
 //          int accumulate)
 const void* dense_f64_kernel_ptr(int n, bool compensated);
+// FP32-product variant (rows pre-scaled by f32prod_scale); same arguments.
+const void* dense_f32p_kernel_ptr(int n);
 int dense_f64_unroll_bits(int n);
 //   args: (const double* partial, int nrec, int stride, int comps, double* out)
 const void* reduce_partials_kernel_ptr();

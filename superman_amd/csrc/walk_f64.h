@@ -78,19 +78,19 @@ __device__ __forceinline__ void static_for(F&& f)
 }
 
 // Product of the first N-1 rows, in four independent chains for ILP.
-template <int N>
-__device__ __forceinline__ double head_product(const double (&x)[N])
+template <int N, class T>
+__device__ __forceinline__ T head_product(const T (&x)[N])
 {
     constexpr int H = N - 1;
     if constexpr (H == 0) {
-        return 1.0;
+        return T(1);
     } else if constexpr (H <= 4) {
-        double p = x[0];
+        T p = x[0];
 #pragma unroll
         for (int r = 1; r < H; ++r) p *= x[r];
         return p;
     } else {
-        double c0 = x[0], c1 = x[1], c2 = x[2], c3 = x[3];
+        T c0 = x[0], c1 = x[1], c2 = x[2], c3 = x[3];
 #pragma unroll
         for (int r = 4; r < H; ++r) {
             if ((r & 3) == 0) c0 *= x[r];
@@ -226,6 +226,7 @@ __device__ __forceinline__ void fma_column(double (&x)[N], const Column<N>& c, d
 // steps, folded into the per-lane (compensated) sum.
 template <int N, bool COMP>
 struct FpTerm {
+    static constexpr bool kCompensated = COMP;
     double t_hi = 0.0, t_lo = 0.0, t_abs = 0.0;
     double blk = 0.0, blk_abs = 0.0;
     __device__ __forceinline__ void add(const double (&x)[N], bool negative)
@@ -240,6 +241,37 @@ struct FpTerm {
         } else {
             t_hi += blk;
         }
+        blk = 0.0;
+        blk_abs = 0.0;
+    }
+};
+
+// FP32 products for nonnegative matrices whose rows the host scaled by
+// powers of two so that every |x_i| < 2^(c+1) with (c+1) N <= 127: x stays
+// FP64 (exact updates), the first N-1 factors are rounded to FP32 and
+// multiplied in FP32 (no overflow by the scaling), and the FP32 product
+// times x_(N-1) is accumulated in the compensated FP64 sums.  Per term the
+// relative error is below (2N-3) 2^-24 to first order (N-1 roundings of x,
+// N-2 of the product), unless the FP32 product falls below 2^-126.
+template <int N>
+struct F32ProdTerm {
+    static constexpr bool kCompensated = true;
+    double t_hi = 0.0, t_lo = 0.0, t_abs = 0.0;
+    double blk = 0.0, blk_abs = 0.0;
+    __device__ __forceinline__ void add(const double (&x)[N], bool negative)
+    {
+        float xf[N];
+#pragma unroll
+        for (int r = 0; r + 1 < N; ++r) xf[r] = (float)x[r];
+        xf[N - 1] = 0.0f;  // not a factor of the head product
+        const double p = (double)head_product<N>(xf);
+        blk = __builtin_fma(negative ? -p : p, x[N - 1], blk);
+        blk_abs = __builtin_fma(__builtin_fabs(p), __builtin_fabs(x[N - 1]), blk_abs);
+    }
+    __device__ __forceinline__ void fold()
+    {
+        dd_add_d(t_hi, t_lo, blk);
+        t_abs += blk_abs;
         blk = 0.0;
         blk_abs = 0.0;
     }

@@ -218,3 +218,16 @@ def test_autotune_keeps_results(engine):
             assert p.exact == q.exact
         else:
             assert abs(p.value - q.value) <= p.err_est + q.err_est
+
+
+def test_rerun_after_a_failed_check_gives_the_same_results():
+    stream = W.serving_stream(60, rate=1.0, seed=21, n_lo=8, n_hi=20)
+    reqs = [(q.kind, q.a, q.rid) for q in stream]
+    with BatchEngine(slots=2, max_requests=8) as eng:
+        rc, first, _, st = eng.replay(reqs, np.zeros(60))
+        assert rc == 0
+        eng.inject_bad_results(5)
+        rc, again, _, st = eng.replay(reqs, np.zeros(60))
+        assert rc == 0 and st.retried >= 5 and st.completed == 60
+        for q, a, b in zip(reqs, first, again):
+            assert _same(to_permanent(a, q[0]), to_permanent(b, q[0]))

@@ -78,3 +78,16 @@ def test_int_walk_selection_rejected_without_gpu():
     ap = a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     assert L.spm_plan_int_walk(ap, 6, _lib.SPM_INT_WALK_FP64, 0, ctypes.byref(plan)) == _lib.SPM_ERANGE
     assert L.spm_plan_int_walk(ap, 6, 7, 0, ctypes.byref(plan)) == _lib.SPM_EINVAL
+
+
+def test_f32_product_mode_rejects_what_it_cannot_serve():
+    # nonnegative real matrices only; checked before any device call
+    L = _lib.lib()
+    r = _lib.SpmResult()
+    a = np.full((5, 5), 0.5)
+    a[2, 3] = -1e-300
+    dp = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert L.spm_dense_f64(dp, 5, _lib.SPM_PREC_F32_PRODUCT, 0, None, ctypes.byref(r)) == _lib.SPM_EINVAL
+    c = np.ones((5, 10))  # complex, interleaved
+    cp = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert L.spm_dense_c128(cp, 5, _lib.SPM_PREC_F32_PRODUCT, 0, None, ctypes.byref(r)) == _lib.SPM_EINVAL

@@ -143,6 +143,21 @@ def test_injected_fault_fails_one_batch_only():
         assert st == 0
 
 
+def test_results_failing_their_checks_run_again():
+    with BatchEngine(dry_run=True, max_requests=4) as eng:
+        eng.inject_bad_results(3)
+        rc, res, _, st = eng.replay(reqs(8), np.zeros(8))
+        assert rc == 0 and all(r.status == 0 for r in res)
+        assert st.retried == 3 and st.completed == 8
+        eng.inject_bad_results(100)  # a request that never passes: two re-runs, then an error
+        rc, res, _, st = eng.replay(reqs(1), np.zeros(1))
+        assert rc == _lib.SPM_EHIP and res[0].status == _lib.SPM_EHIP and st.retried == 2
+        assert eng.last_error() == "injected bad result"
+        eng.inject_bad_results(0)
+        rc, res, _, st = eng.replay(reqs(4), np.zeros(4))
+        assert rc == 0 and st.retried == 0
+
+
 def test_replay_latency_accounting():
     stream = W.serving_stream(200, rate=2e4, seed=5, n_lo=8, n_hi=16)
     r = [(q.kind, q.a, q.rid) for q in stream]
