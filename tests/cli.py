@@ -18,14 +18,16 @@ def binary():
     return BIN
 
 
-def write_mtx(path, a, pattern=False):
+def write_mtx(path, a, pattern=False, symmetric=False):
+    """MatrixMarket coordinate file; `symmetric` writes the lower triangle
+    under a symmetric banner (read back with undirected=true)."""
     n = a.shape[0]
     cplx = np.iscomplexobj(a)
     rows = []
     for i in range(n):
         for j in range(n):
             v = a[i, j]
-            if v == 0:
+            if v == 0 or (symmetric and j > i):
                 continue
             if pattern:
                 rows.append(f"{i + 1} {j + 1}")
@@ -34,15 +36,17 @@ def write_mtx(path, a, pattern=False):
             else:
                 rows.append(f"{i + 1} {j + 1} {v:.17g}")
     kind = "pattern" if pattern else ("complex" if cplx else "real")
+    shape = "symmetric" if symmetric else "general"
     with open(path, "w") as f:
-        f.write(f"%%MatrixMarket matrix coordinate {kind} general\n{n} {n} {len(rows)}\n" + "\n".join(rows) + "\n")
+        f.write(f"%%MatrixMarket matrix coordinate {kind} {shape}\n% written by the tests\n{n} {n} {len(rows)}\n" +
+                "\n".join(rows) + "\n")
 
 
-def run(a=None, *args, pattern=False, path=None):
+def run(a=None, *args, pattern=False, symmetric=False, path=None):
     with tempfile.TemporaryDirectory(prefix="spm_cli_") as d:
         if path is None:
             path = os.path.join(d, "m.mtx")
-            write_mtx(path, a, pattern)
+            write_mtx(path, a, pattern, symmetric)
         p = subprocess.run([binary(), f"filename={path}", *args], capture_output=True, text=True, timeout=600)
     return p.returncode, p.stdout, p.stderr
 

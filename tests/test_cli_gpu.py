@@ -1,5 +1,6 @@
 """The command-line driver on the GPU: main.cpp's flow (DM pruning,
 components, algorithm selection, per-component product) end to end."""
+import json
 import math
 import os
 
@@ -9,6 +10,7 @@ import pytest
 import cli
 from conftest import GOLDEN_DIR, rel_err
 from oracle import oracle as O
+import superman_amd as S
 from superman_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -51,3 +53,33 @@ def test_undirected_mirrors_entries():
     rc, out, err = cli.run(a, "undirected=true")
     assert rc == 0, err
     assert cli.value(out) == float(math.factorial(8))
+
+
+EXAMPLES = json.load(open(os.path.join(GOLDEN_DIR, "example_golden.json")))["cases"]
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+def test_reference_example_matrices(name):
+    # ExampleMatrices/<name>.mtx as its banner describes it; the reference's
+    # own CPU binary's permanent on the same input
+    c = EXAMPLES[name]
+    a = np.load(os.path.join(GOLDEN_DIR, c["file"]), allow_pickle=False)
+    args = [f"binary={'true' if c['binary'] else 'false'}", f"undirected={'true' if c['undirected'] else 'false'}"]
+    rc, out, err = cli.run(a, *args, pattern=c["binary"], symmetric=c["undirected"])
+    assert rc == 0, err
+    v = cli.value(out)
+    if c["binary"]:  # integer result below 2^53: exact in both
+        assert v == c["value"]
+    else:
+        # d_ss is ill-conditioned (entries from 1e-5 to 2.5e4): FP64 results
+        # carry errors of order cond * n * 2^-53, the reference's too, so
+        # the check is against the leaves' double-double permanent within
+        # the library's own bound for this input
+        r = S.permanent_decomposed(a)
+        truth = c["leaves_dd"]
+        print(f"{name}: cli {v!r} library {r.value!r} dd {truth!r} reference {c['value']!r} "
+              f"err_est/|dd| {r.err_est / abs(truth):.2e}")
+        assert abs(r.value - truth) <= r.err_est
+        assert abs(v - truth) <= r.err_est
+        # compensated sums: closer to the truth than the reference's FP64 value
+        assert abs(v - truth) < abs(c["value"] - truth)

@@ -122,3 +122,18 @@ def test_bad_arguments():
         api.decompose(np.full((3, 3), np.nan))
     with pytest.raises(_lib.SpmError):
         api.decompose(np.ones((3, 3)), 64)
+
+
+EXAMPLES = json.load(open(os.path.join(GOLDEN_DIR, "example_golden.json")))["cases"]
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES), ids=str)
+def test_example_matrix_leaves_match_reference(name):
+    c = EXAMPLES[name]
+    a = np.load(os.path.join(GOLDEN_DIR, c["file"]), allow_pickle=False)
+    d = api.decompose(a)
+    assert len(d.leaves) == c["leaves"]
+    assert float(f"{d.avg_n:.6g}") == c["avg_n"]
+    assert float(f"{d.avg_nnz:.6g}") == c["avg_nnz"]
+    if c["binary"]:  # 0/1: the exact permanent from the leaves, as the reference printed it
+        assert exact_from_leaves(d) == int(c["value"])
