@@ -102,6 +102,26 @@ def measured_traffic(kernel: str):
     return t["bytes_per_launch"] if t.get("kernel") == kernel else None
 
 
+def measured_counters(kernel: str, kernel_ms: float):
+    """Counter-based figures for the dominant kernel from the committed PMC
+    run (instruction and FP64-flop counts per launch do not depend on the
+    box), turned into rates with this run's kernel time."""
+    path = os.path.join(ROOT, "profiles", "dense_f64_n40_counters.json")
+    try:
+        with open(path) as f:
+            c = json.load(f)
+    except OSError:
+        return None
+    if c.get("kernel") != kernel or kernel_ms <= 0:
+        return None
+    t = kernel_ms * 1e-3
+    counted = c["counted_fp64_flops_per_launch"] / t / 1e12
+    issue = c["sq_insts_valu_per_launch"] * 4.0 / (1024 * 2.4e9 * t)  # 4 cycles per wave64 FP64 op, 1024 SIMDs
+    return {"counted_tflops": counted, "counted_frac": counted / PEAK_FP64_TFLOPS,
+            "counted_rule": c["counted_flops_rule"], "valu_issue_frac": issue,
+            "valu_busy_pmc": c["valu_busy"], "source": c["source"]}
+
+
 def serving(count: int, seed: int = 2026, load: float = 0.8) -> dict:
     from superman_amd import api
     from superman_amd import workloads as W
@@ -273,7 +293,8 @@ def main(argv=None):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(info["kernel"]),
                      "kernel": info["kernel"], "kernel_ms": k_ms,
-                     "flops_per_launch": flops, "flop_convention": "2n per subset (add, mul, accumulate; FMA = 2)"},
+                     "flops_per_launch": flops, "flop_convention": "2n per subset (add, mul, accumulate; FMA = 2)",
+                     "counters": measured_counters(info["kernel"], k_ms)},
         "result": {"value": r.value, "err_est": r.err_est, "cond": r.cond,
                    f"{other}_value": r_other.value, f"{other}_kernel_ms": r_other.kernel_ms},
         "geometry": info,
