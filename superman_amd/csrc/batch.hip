@@ -29,7 +29,8 @@
 //             is below 2^(bit width of n), e.g. 0/1 matrices)
 // and in every record word 7 = batch_tag(seq, item or request index); in a
 // result record word 6 = the number of item records the reduction found
-// without this batch's tag (0 unless the records were not yet visible).
+// without this batch's tag (0 unless the records were not yet visible) and
+// word 5 = record_check() of the other seven words.
 #include "spm_batch.h"
 #include "walk_c128.h"
 #include "walk_f64.h"
@@ -186,6 +187,10 @@ __device__ __forceinline__ void reduce_request(int kind, const double* __restric
     for (int off = 32; off >= 1; off >>= 1) stale += __shfl_xor(stale, off, kWave);
     o[kBatchStale] = (double)stale;
     o[kBatchTag] = __longlong_as_double((long long)batch_tag(seq, r));
+    uint64_t w[kBatchRec];
+#pragma unroll
+    for (int c = 0; c < kBatchRec; ++c) w[c] = (uint64_t)__double_as_longlong(o[c]);
+    o[kBatchCheck] = __longlong_as_double((long long)record_check(w));
 #pragma unroll
     for (int c = 0; c < kBatchRec; ++c) out[c] = o[c];
 }

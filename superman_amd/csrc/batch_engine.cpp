@@ -474,14 +474,15 @@ void fail_slot(spm_batch* b, Slot& s, int status, double now)
 }
 
 // A request's result is on the host: its flag holds the slot's batch number
-// and its record carries its own tag (the record's words are stored before
-// the flag, but each read here re-checks both).
+// and its record is whole -- its own tag and a matching check word (the
+// record's words are stored before the flag, but each read here checks
+// them all).
 bool published(const Slot& s, size_t j)
 {
     if (__atomic_load_n(s.h_flag + j, __ATOMIC_ACQUIRE) != s.seq) return false;
-    uint64_t tag;
-    std::memcpy(&tag, s.h_res + j * kBatchRec + kBatchTag, sizeof tag);
-    return tag == batch_tag(s.seq, (uint32_t)j);
+    uint64_t w[kBatchRec];
+    std::memcpy(w, s.h_res + j * kBatchRec, sizeof w);
+    return w[kBatchTag] == batch_tag(s.seq, (uint32_t)j) && w[kBatchCheck] == record_check(w);
 }
 
 // Retires finished slots; returns how many were retired.

@@ -12,6 +12,7 @@ constexpr int kBatchComplex = 2;    // compensated complex128
 constexpr int kBatchInt = 3;        // exact integer (mod 2^128), int32 walk
 constexpr int kBatchIntSmall = 4;   // exact integer, small row sums: FP64 walk
 constexpr int kBatchRec = 8;        // doubles per item / result record
+constexpr int kBatchCheck = 5;      // result word: record_check() of the other seven words
 constexpr int kBatchStale = 6;      // result word: item records of another batch seen by the reduction
 constexpr int kBatchTag = 7;        // item / result word: batch_tag(seq, item or request index)
 
@@ -24,6 +25,20 @@ __host__ __device__
 inline uint64_t batch_tag(uint32_t seq, uint32_t idx)
 {
     return ((uint64_t)seq << 32) | idx;
+}
+
+// Check word of a result record: an FNV-1a style mix of its other words'
+// bit patterns, so that a reader can tell a whole record from one whose
+// stores have not all arrived.
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline uint64_t record_check(const uint64_t* w)
+{
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (int c = 0; c < kBatchRec; ++c)
+        if (c != kBatchCheck) h = (h ^ w[c]) * 0x100000001b3ull;
+    return h;
 }
 constexpr int kBatchAlign = 64;     // byte alignment of every arena region
 
