@@ -77,3 +77,16 @@ def test_haar_n32_plain_vs_compensated(golden):
         d = c["dd"]
         ref = complex(d[0] + d[1], d[2] + d[3])
         assert rel_err(comp.value, ref) <= TOL
+
+
+@pytest.mark.parametrize("n", [33, 36])
+def test_orders_with_three_column_parts(n):
+    # orders 33..48 take each column in three parts of at most 32 doubles
+    r = api.dense_c128(np.full((n, n), 1j))
+    exact = (1j) ** n * math.factorial(n)
+    assert abs(r.value - exact) <= r.err_est and rel_err(r.value, exact) <= TOL
+    b = W.dense_normal(n, n)
+    rc = api.dense_c128(b.astype(np.complex128))
+    rr = api.dense_f64(b)
+    assert abs(rc.value - rr.value) <= rc.err_est + rr.err_est
+    assert abs(rc.value.imag) == 0.0
