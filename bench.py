@@ -197,6 +197,12 @@ def secondary() -> dict:
         kname = p.info()["kernel"]
     out["sparse01_n40_codegen"] = {"kernel_ms": float(np.mean([r.kernel_ms for r in rs])), "exact": str(rs[-1].exact),
                                    "compile_s": info["compile_seconds"], "kernel": kname}
+    with S.DensePlan(b, "exact", int_walk="sparse") as p:
+        p.run()
+        rs = p.run()
+        out["sparse01_n40_sparse_walk"] = {"kernel_ms": rs.kernel_ms, "exact": str(rs.exact),
+                                           "kernel": p.info()["kernel"],
+                                           "walk": "compressed columns read at run time (no code generation)"}
     d = S.permanent_decomposed(b.astype(np.float64))  # warm (first-call code loading)
     d = S.permanent_decomposed(b.astype(np.float64))
     out["sparse01_n40_decomposed"] = {"seconds": d.seconds, "kernel_ms": d.kernel_ms, "exact": str(d.exact),

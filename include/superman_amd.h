@@ -141,13 +141,18 @@ int spm_plan_dense_f64(const double* a, int n, int precision, int device, spm_pl
 int spm_plan_dense_c128(const double* a, int n, int precision, int device, spm_plan** plan);
 int spm_plan_int(const int64_t* a, int n, int device, spm_plan** plan);
 /* The same with the integer walk forced: SPM_INT_WALK_INT32 (any entries in
- * range) or SPM_INT_WALK_FP64 (exact FP64 walk with grouped products; only
+ * range), SPM_INT_WALK_FP64 (exact FP64 walk with grouped products; only
  * when every absolute row sum is below 2^b, b = bit width of n -- all 0/1
- * matrices -- else SPM_ERANGE).  spm_int_exact / spm_plan_int choose the
- * FP64 walk whenever it applies. */
+ * matrices -- else SPM_ERANGE) or SPM_INT_WALK_SPARSE (compressed columns:
+ * per step only the flipped column's rows, a zero-row count, the product
+ * only when no row is zero; the reference's runtime-CSC sparse kernels,
+ * SparseKernelDefinitions.cuh:34-509).  SPM_INT_WALK_AUTO takes the sparse
+ * walk for sparse matrices and otherwise the FP64 walk whenever it
+ * applies. */
 #define SPM_INT_WALK_AUTO   0
 #define SPM_INT_WALK_INT32  1
 #define SPM_INT_WALK_FP64   2
+#define SPM_INT_WALK_SPARSE 3
 int spm_plan_int_walk(const int64_t* a, int n, int walk, int device, spm_plan** plan);
 /* Compiles (or takes from the cache) the generated kernel for `a`. */
 int spm_plan_codegen(const int64_t* a, int n, int options, int device, spm_plan** plan);

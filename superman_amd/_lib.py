@@ -34,6 +34,7 @@ SPM_PREC_F32_PRODUCT = 2
 SPM_INT_WALK_AUTO = 0
 SPM_INT_WALK_INT32 = 1
 SPM_INT_WALK_FP64 = 2
+SPM_INT_WALK_SPARSE = 3
 
 SPM_DECOMP_DM = 1
 SPM_DECOMP_COMPONENTS = 2

@@ -138,7 +138,8 @@ class DensePlan:
     """A permanent with the matrix resident on the device: FP64, complex128
     for a complex matrix, or exact integer with precision="exact"."""
 
-    INT_WALKS = {"auto": _lib.SPM_INT_WALK_AUTO, "int32": _lib.SPM_INT_WALK_INT32, "fp64": _lib.SPM_INT_WALK_FP64}
+    INT_WALKS = {"auto": _lib.SPM_INT_WALK_AUTO, "int32": _lib.SPM_INT_WALK_INT32, "fp64": _lib.SPM_INT_WALK_FP64,
+                 "sparse": _lib.SPM_INT_WALK_SPARSE}
 
     def __init__(self, a, precision="kahan", device: int = 0, int_walk: str = "auto"):
         self.complex = np.iscomplexobj(a)

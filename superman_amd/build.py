@@ -37,6 +37,7 @@ SOURCES = [
     ("dense_c128.hip", []),
     ("int_dense.hip", []),
     ("int_small.hip", []),
+    ("int_sparse.hip", []),
     ("batch.hip", []),
     ("capi.cpp", []),
     ("prep.cpp", []),

@@ -270,14 +270,17 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
 {
     if (!out) return SPM_EINVAL;
     *out = nullptr;
-    if (walk != SPM_INT_WALK_AUTO && walk != SPM_INT_WALK_INT32 && walk != SPM_INT_WALK_FP64) return SPM_EINVAL;
+    if (walk != SPM_INT_WALK_AUTO && walk != SPM_INT_WALK_INT32 && walk != SPM_INT_WALK_FP64 &&
+        walk != SPM_INT_WALK_SPARSE)
+        return SPM_EINVAL;
     std::vector<int> y0;
     double lb = 0;
     int rc = int_prepare(a, n, y0, &lb);
     if (rc != SPM_OK) return rc;
     const bool small = int_small_ok(a, n);
     if (walk == SPM_INT_WALK_FP64 && !small) return SPM_ERANGE;
-    const bool fp = walk == SPM_INT_WALK_FP64 || (walk == SPM_INT_WALK_AUTO && small);
+    const bool sp = walk == SPM_INT_WALK_SPARSE || (walk == SPM_INT_WALK_AUTO && int_sparse_preferred(a, n));
+    const bool fp = !sp && (walk == SPM_INT_WALK_FP64 || (walk == SPM_INT_WALK_AUTO && small));
     int cus = 0;
     int st = device_cus(device, &cus);
     if (st != SPM_OK) return st;
@@ -290,7 +293,7 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
     p->precision = -1;
     p->device = device;
     p->log2_bound = lb;
-    p->kern = fp ? int_small_kernel_ptr(n) : int_dense_kernel_ptr(n);
+    p->kern = sp ? int_sparse_kernel_ptr(n) : fp ? int_small_kernel_ptr(n) : int_dense_kernel_ptr(n);
     p->rec = 2;
     p->comps = 0;
     if (!p->kern) {
@@ -310,10 +313,20 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
     const double est_s = std::ldexp(1.0, n - 1) * ((fp ? 5.0 : 12.0) * n) / 3.9e13;
     p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
     char kn[96];
-    std::snprintf(kn, sizeof kn, fp ? "int_small_kernel<%d>" : "int_dense_kernel<%d>", n);
+    std::snprintf(kn, sizeof kn, sp ? "int_sparse_kernel<%d>" : fp ? "int_small_kernel<%d>" : "int_dense_kernel<%d>",
+                  n);
     p->kname = kn;
     const size_t nn = (size_t)n * n;
-    if (fp) {
+    if (sp) {
+        std::vector<int> blob;
+        int_sparse_blob(a, n, blob);
+        SPM_TRY(hipMalloc(&p->d_M, sizeof(int) * blob.size()));
+        std::vector<int> ys(y0.begin(), y0.end());
+        ys.push_back(small ? 1 : 0);  // products in FP64 groups (int_sparse_kernel)
+        SPM_TRY(hipMalloc(&p->d_x0, sizeof(int) * ys.size()));
+        SPM_TRY(hipMemcpy(p->d_M, blob.data(), sizeof(int) * blob.size(), hipMemcpyHostToDevice));
+        SPM_TRY(hipMemcpy(p->d_x0, ys.data(), sizeof(int) * ys.size(), hipMemcpyHostToDevice));
+    } else if (fp) {
         std::vector<double> colmaj(2 * nn), yd(y0.begin(), y0.end());
         int_small_planes(a, n, colmaj.data());
         SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * nn));

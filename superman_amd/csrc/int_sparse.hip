@@ -1,0 +1,84 @@
+// int_sparse.hip -- single-matrix exact Ryser kernel for sparse integer
+// matrices in compressed-column form (gfx950).  The per-chunk walk is in
+// walk_sparse.h; this file holds the grid-strided task loop over one
+// matrix's chunks and the block sum mod 2^128 into per-block records, which
+// int_dense.hip's reduce_i128_kernel sums.
+#include "walk_sparse.h"
+
+#ifndef SPM_SPARSE_ONLY_N
+#define SPM_SPARSE_ONLY_N 0
+#endif
+
+namespace spm {
+
+namespace sparse {
+
+template <int N>
+__global__ void __launch_bounds__(kDenseBlock)
+int_sparse_kernel(const int* __restrict__ blob,                // compressed columns (walk_sparse.h)
+                  const int* __restrict__ y0,                  // y(empty set), N values
+                  unsigned long long* __restrict__ partial,    // [gridDim.x][2]: T mod 2^128 (lo, hi)
+                  uint64_t task_begin, uint64_t task_end, uint64_t num_chunks, int L, int accumulate)
+{
+    __shared__ unsigned long long red[2 * (kDenseBlock / kWave)];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = (uint64_t)blockIdx.x * (kDenseBlock / kWave) + (threadIdx.x / kWave);
+    const uint64_t nwave = (uint64_t)gridDim.x * (kDenseBlock / kWave);
+    unsigned __int128 acc = 0;
+
+    for (uint64_t task = task_begin + wave; task < task_end; task += nwave) {
+        const uint64_t h = task * kWave + lane;
+        unsigned __int128 t = 0;
+        walk_sparse<N>(blob, y0, h, L, t);
+        if (h < num_chunks) acc += t;  // lanes past the end walk a dummy chunk
+    }
+
+    // Block sum mod 2^128 (fixed order: wave butterfly, then waves in order).
+    unsigned long long lo = (unsigned long long)acc, hi = (unsigned long long)(acc >> 64);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long olo = __shfl_xor(lo, off, kWave);
+        const unsigned long long ohi = __shfl_xor(hi, off, kWave);
+        const unsigned long long s = lo + olo;
+        hi = hi + ohi + (s < lo ? 1ull : 0ull);
+        lo = s;
+    }
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[2 * w] = lo;
+        red[2 * w + 1] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned __int128 t = 0;
+        for (int i = 0; i < kDenseBlock / kWave; ++i) t += ((unsigned __int128)red[2 * i + 1] << 64) | red[2 * i];
+        unsigned long long* out = partial + 2 * blockIdx.x;
+        if (accumulate) t += ((unsigned __int128)out[1] << 64) | out[0];
+        out[0] = (unsigned long long)t;
+        out[1] = (unsigned long long)(t >> 64);
+    }
+}
+
+template <int N>
+const void* sparse_ptr()
+{
+    if constexpr (SPM_SPARSE_ONLY_N != 0 && N != SPM_SPARSE_ONLY_N) return nullptr;
+    else return reinterpret_cast<const void*>(&int_sparse_kernel<N>);
+}
+
+template <int... Is>
+const void* sparse_lookup(int n, std::integer_sequence<int, Is...>)
+{
+    static const void* const tab[] = {nullptr, sparse_ptr<Is + 1>()...};
+    return tab[n];
+}
+
+}  // namespace sparse
+
+const void* int_sparse_kernel_ptr(int n)
+{
+    if (n < 1 || n > kMaxN) return nullptr;
+    return sparse::sparse_lookup(n, std::make_integer_sequence<int, kMaxN>{});
+}
+
+}  // namespace spm

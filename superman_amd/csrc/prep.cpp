@@ -6,6 +6,7 @@
 #include "spm_host.h"
 #include "spm_kernels.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -71,6 +72,62 @@ void int_planes(const int64_t* a, int n, int* planes)
             planes[(size_t)j * n + i] = (int)(2 * a[i * n + j]);
             planes[nn + (size_t)j * n + i] = (int)(-2 * a[i * n + j]);
         }
+}
+
+// Compressed columns of +2A / -2A as the records of spm_kernels.h.
+void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
+{
+    std::vector<std::vector<int>> rows(n), vals(n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i)
+            if (a[(size_t)i * n + j] != 0) {
+                rows[j].push_back(i);
+                vals[j].push_back((int)(2 * a[(size_t)i * n + j]));
+            }
+    int extra = 0;
+    for (int j = 0; j < n; ++j) {
+        const int k = (int)rows[j].size();
+        extra += k > kSparseSlots ? (k - 1) / kSparseSlots : 0;
+    }
+    const int nrec = 2 * n + 2 * extra;
+    blob.assign((size_t)nrec * kSparseRec, 0);
+    int next_free = 2 * n;
+    for (int j = 0; j < n; ++j)
+        for (int plane = 0; plane < 2; ++plane) {
+            const int k = (int)rows[j].size();
+            int rec = 2 * j + plane;
+            for (int e0 = 0;; e0 += kSparseSlots) {
+                int* r = blob.data() + (size_t)rec * kSparseRec;
+                for (int u = 0; u < kSparseSlots; ++u) {
+                    const int e = e0 + u;
+                    r[1 + u] = e < k ? rows[j][e] : n;
+                    r[1 + kSparseSlots + u] = e < k ? (plane ? -vals[j][e] : vals[j][e]) : 0;
+                }
+                r[kSparseRec - 1] = std::min(kSparseSlots, k - e0);
+                if (e0 + kSparseSlots >= k) {
+                    r[0] = -1;
+                    break;
+                }
+                r[0] = next_free;
+                rec = next_free++;
+            }
+        }
+}
+
+// Where the sparse walk is the faster one (measured on MI355X): matrices
+// with at most a quarter of their entries nonzero, from order 26 (below, a
+// permanent is too small a job for its per-chunk start), and -- when the
+// FP64 walk applies (small row sums) -- from order 34 (config-3 style 0/1
+// leaves of order 30 run equally fast on both; the whole order-40 matrix
+// 1.7x faster on the sparse walk).  With larger entries it replaces the
+// int32 walk (1.13x faster on order-30 leaves of the curtis54 example).
+bool int_sparse_preferred(const int64_t* a, int n)
+{
+    if (n < 26) return false;
+    size_t nnz = 0;
+    for (size_t e = 0; e < (size_t)n * n; ++e) nnz += a[e] != 0 ? 1 : 0;
+    if (4 * nnz > (size_t)n * n) return false;
+    return int_small_ok(a, n) ? n >= 34 : true;
 }
 
 // True when every absolute row sum is below 2^int_small_bits(n), so that

@@ -52,6 +52,8 @@ inline double f32prod_err_est(int n, double abs_sum)
 void dense_prepare(const double* a, int n, int w, double* planes, double* x0);
 int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound);
 void int_planes(const int64_t* a, int n, int* planes);
+void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob);
+bool int_sparse_preferred(const int64_t* a, int n);
 bool int_small_ok(const int64_t* a, int n);
 void int_small_planes(const int64_t* a, int n, double* planes);
 void dense_fill_result(int n, bool complex_kind, bool compensated, const double* rec, spm_result* r);

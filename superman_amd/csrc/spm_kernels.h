@@ -43,6 +43,22 @@ const void* reduce_i128_kernel_ptr();
 const void* int_small_kernel_ptr(int n);
 int int_small_bits(int n);
 
+// int_sparse.hip -- exact walk over compressed columns (sparse integer
+// matrices); same arguments as int_dense_kernel except that M is the record
+// array below and y0 is n + 1 int32 values: y(empty set), then 1 when every
+// absolute row sum is below 2^b, b = bit width of n (products may then be
+// formed in FP64 groups, as int_small_kernel does), else 0.  A column's
+// entries in one plane (+2A or -2A) sit in 16-int records (64 bytes, one
+// scalar load):
+//   [0] next record of the column (or -1), [1..7] rows, [8..14] values,
+//   [15] number of slots in use
+// Record 2j + p (p = 0: +2A, 1: -2A) is the first of column j; longer
+// columns continue in records after the first 2n.  Unused slots hold row n
+// (a spare register of the walk) and value 0.
+const void* int_sparse_kernel_ptr(int n);
+constexpr int kSparseRec = 16;   // ints per record
+constexpr int kSparseSlots = 7;  // entries per record
+
 // Generated SpaRyser kernels walk 2^3-step blocks with L > 3.
 constexpr int kCodegenMinN = 5;
 

@@ -1,0 +1,200 @@
+// walk_sparse.h -- exact Gray-code Ryser walk for sparse integer matrices on
+// gfx950, with the matrix's columns read at run time.
+//
+// Replaces the reference's runtime-CSC sparse kernels
+// (Algorithms/GPU/Sparse/SparseKernelDefinitions.cuh:34-509 driven by
+// spSingleGPU.cuh:44-234): per Gray step only the rows of the flipped
+// column change, a count of zero rows is kept, and the product over all
+// rows is formed only when no row is zero.  As in the dense walks here,
+// every lane of a wavefront walks its own aligned chunk of 2^L steps and
+// all lanes flip the same column in lockstep, so a column's rows and values
+// are wave-uniform: a column comes in as 64-byte records (spm_kernels.h),
+// one scalar load each, prefetched one step ahead, and a row index selects
+// the register of y by VGPR indexing (the index is an SGPR), so y stays in
+// VGPRs.  A record's unused slots name the spare register y[N] with value 0,
+// so every record is applied without branches.  Arithmetic is exact (y = 2x
+// in int32, T mod 2^128, as walk_int.h).
+#ifndef SPM_WALK_SPARSE_H
+#define SPM_WALK_SPARSE_H
+
+#include "spm_device.h"
+#include "spm_kernels.h"
+#include "walk_int.h"
+#include "walk_intfp.h"
+
+namespace spm {
+namespace sparse {
+
+using ints::const_i32;
+using ints::fence;
+using ints::step_ptr;
+
+struct Rec {
+    int v[kSparseRec];
+};
+
+__device__ __forceinline__ void load_rec(Rec& r, const int* recs, int idx)
+{
+    const const_i32* __restrict__ q = step_ptr(recs + idx * kSparseRec);
+#pragma unroll
+    for (int i = 0; i < kSparseRec; ++i) r.v[i] = q[i];
+}
+
+// The record of column j in plane p (0: +2A, 1: -2A).
+__device__ __forceinline__ int rec_index(int j, int p) { return 2 * j + p; }
+
+// y[row] += value for the slots in use of one record (a uniform count:
+// the skipped slots cost a scalar branch, not the vector update), keeping
+// Z = number of zero rows (the spare y[N] is never zero).  MODE 0: values
+// as stored; 1: negated on lanes with m = -1 (per-lane direction); 2: kept
+// on lanes with m = -1 only (chunk-start membership).
+template <int N, int MODE>
+__device__ __forceinline__ void apply_slots(int (&y)[N + 1], int& Z, const Rec& r, int m)
+{
+    const int used = r.v[kSparseRec - 1];
+#pragma unroll
+    for (int u = 0; u < kSparseSlots; ++u) {
+        if (u < used) {
+            const int row = r.v[1 + u];
+            int v = r.v[1 + kSparseSlots + u];
+            if constexpr (MODE == 1) v = (v ^ m) - m;
+            else if constexpr (MODE == 2) v &= m;
+            const int old = y[row];
+            const int now = old + v;
+            y[row] = now;
+            Z += (now == 0) - (old == 0);
+        }
+    }
+}
+
+// A whole column from its first record (already loaded); longer columns
+// continue in further records, loaded here (rare for sparse inputs).
+template <int N, int MODE>
+__device__ __forceinline__ void apply_column(int (&y)[N + 1], int& Z, const Rec& first, const int* recs, int m)
+{
+    apply_slots<N, MODE>(y, Z, first, m);
+    int nx = first.v[0];
+    while (nx >= 0) {
+        Rec r;
+        load_rec(r, recs, nx);
+        apply_slots<N, MODE>(y, Z, r, m);
+        nx = r.v[0];
+    }
+}
+
+// acc +/-= prod_r y_r (mod 2^128) on lanes with no zero row; the wave takes
+// the product branch only when one of its lanes needs it.  SMALL (uniform:
+// every |y_r| < 2^b, b = bit width of N) forms the product as walk_intfp.h
+// does -- FP64 groups of exact integers below 2^51, then int64 -- which
+// costs far fewer multiplies than the int64 row pairs of the general case.
+template <int N>
+__device__ __forceinline__ void add_term(const int (&y)[N + 1], int Z, bool negative, bool small,
+                                         unsigned __int128& acc)
+{
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(Z == 0) != 0, 0)) {
+        if (Z == 0) {
+            unsigned __int128 p;
+            if (small) {
+                constexpr int G = real::kIntFpGroup<N>;
+                constexpr int NG = real::kIntFpGroups<N>;
+#pragma unroll
+                for (int q = 0; q < NG; ++q) {
+                    const int r0 = q * G;
+                    const int r1 = (r0 + G < N) ? r0 + G : N;
+                    double g = (double)y[r0];
+#pragma unroll
+                    for (int r = r0 + 1; r < r1; ++r) g *= (double)y[r];
+                    const unsigned __int128 w = (unsigned __int128)(__int128)real::exact_int64(g);
+                    p = q == 0 ? w : p * w;
+                }
+            } else {
+                p = (unsigned __int128)(__int128)y[0];
+#pragma unroll
+                for (int r = 1; r + 1 < N; r += 2) {
+                    const long long q = (long long)y[r] * (long long)y[r + 1];
+                    p *= (unsigned __int128)(__int128)q;
+                }
+                if constexpr (N % 2 == 0) p *= (unsigned __int128)(__int128)y[N - 1];
+            }
+            acc = negative ? acc - p : acc + p;
+        }
+    }
+}
+
+// One lane's chunk: the 2^L Gray steps starting at k = h 2^L, added into
+// acc (mod 2^128).  Steps as in walk_int.h: k = 2m flips column ctz(m)+1
+// in a uniform direction, k = 2m+1 flips column 0, block nblk/2 flips
+// column L-1 in a per-lane direction.
+template <int N>
+__device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const int* __restrict__ y0, uint64_t h,
+                                            int L, unsigned __int128& acc)
+{
+    int y[N + 1];
+    {
+        const const_i32* __restrict__ ys = step_ptr(y0);
+#pragma unroll
+        for (int r = 0; r < N; ++r) y[r] = ys[r];
+        y[N] = 1;
+    }
+    const bool small = ((const const_i32*)y0)[N] != 0;
+    int Z = 0;
+    const uint64_t k0 = h << L;
+    const uint64_t g0 = k0 ^ (k0 >> 1);
+    for (int j = (L > 0 ? L - 1 : 0); j < N - 1; ++j) {
+        Rec r;
+        load_rec(r, recs, rec_index(j, 0));
+        apply_column<N, 2>(y, Z, r, recs, -(int)((g0 >> j) & 1u));
+    }
+    Z = 0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) Z += y[r] == 0 ? 1 : 0;
+    const int lane_mask = -(int)(h & 1u);  // direction of column L-1: -1 = remove
+
+    if constexpr (N == 1) {
+        add_term<N>(y, Z, false, small, acc);
+    } else if constexpr (N == 2) {  // L = 1
+        add_term<N>(y, Z, false, small, acc);
+        Rec r;
+        load_rec(r, recs, rec_index(0, 0));
+        apply_column<N, 1>(y, Z, r, recs, lane_mask);
+        add_term<N>(y, Z, true, small, acc);
+    } else {
+        const uint32_t nblk = 1u << (L - 1);
+        const uint32_t half = nblk >> 1;
+        auto a_rec = [&](uint32_t m) __attribute__((always_inline)) {  // record of step k = 2m
+            const int tz = __builtin_ctz(m);
+            return rec_index(tz + 1, (int)((m >> (tz + 1)) & 1u));
+        };
+        Rec ra, rb;
+        add_term<N>(y, Z, false, small, acc);  // k = 0
+        load_rec(rb, recs, rec_index(0, 0));
+        apply_column<N, 0>(y, Z, rb, recs, 0);  // k = 1: +column 0
+        load_rec(ra, recs, a_rec(1));
+        fence();
+        add_term<N>(y, Z, true, small, acc);
+        fence();
+        auto block = [&](uint32_t m, auto SPECIAL) __attribute__((always_inline)) {
+            if constexpr (decltype(SPECIAL)::value) apply_column<N, 1>(y, Z, ra, recs, lane_mask);
+            else apply_column<N, 0>(y, Z, ra, recs, 0);
+            load_rec(rb, recs, rec_index(0, (int)(m & 1u)));  // step 2m+1: column 0
+            fence();
+            add_term<N>(y, Z, false, small, acc);
+            fence();
+            apply_column<N, 0>(y, Z, rb, recs, 0);
+            load_rec(ra, recs, a_rec(m + 1));  // m+1 = nblk: column L <= N-1, a valid record
+            fence();
+            add_term<N>(y, Z, true, small, acc);
+            fence();
+        };
+#pragma nounroll
+        for (uint32_t m = 1; m < half; ++m) block(m, std::false_type{});
+        block(half, std::true_type{});
+#pragma nounroll
+        for (uint32_t m = half + 1; m < nblk; ++m) block(m, std::false_type{});
+    }
+}
+
+}  // namespace sparse
+}  // namespace spm
+
+#endif

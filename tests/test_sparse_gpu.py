@@ -1,0 +1,78 @@
+"""Exact sparse walk over compressed columns (SPM_INT_WALK_SPARSE): bit-exact
+against the int128 oracle and against the two dense integer walks, on 0/1
+and small-integer sparse matrices, zero rows and columns, and the config-3
+generator at full size."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_DIR
+from oracle import oracle as O
+from superman_amd import _lib, api
+from superman_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def sparse_ints(n, seed, per_row=4, vmax=3, signed=True):
+    rng = np.random.default_rng(seed)
+    a = np.zeros((n, n), dtype=np.int64)
+    p = rng.permutation(n)
+    for i in range(n):
+        a[i, p[i]] = rng.integers(1, vmax + 1)
+        k = min(int(rng.integers(0, per_row)), n - 1)
+        if k == 0:
+            continue
+        cols = rng.choice([j for j in range(n) if j != p[i]], k, replace=False)
+        lo = -vmax if signed else 1
+        a[i, cols] = rng.integers(lo, vmax + 1, k)
+    return a
+
+
+def run(a, walk):
+    with api.DensePlan(a, "exact", int_walk=walk) as p:
+        assert p.info()["kernel"].startswith({"sparse": "int_sparse", "int32": "int_dense", "fp64": "int_small"}[walk])
+        return p.run().exact
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 12, 17, 20, 24])
+def test_vs_oracle(n):
+    for seed, vmax, signed in ((n, 1, False), (100 + n, 3, True)):
+        a = sparse_ints(n, seed, per_row=4, vmax=vmax, signed=signed)
+        ref, st = O.int_exact(a)
+        assert st == 0
+        assert run(a, "sparse") == ref, (n, seed)
+
+
+def test_dense_input_and_zero_lines():
+    a = (np.random.default_rng(3).random((12, 12)) < 0.5).astype(np.int64)
+    ref, _ = O.int_exact(a)
+    assert run(a, "sparse") == ref
+    b = sparse_ints(14, 9)
+    b[:, 6] = 0  # a zero column: the permanent is 0, the walk still runs every step
+    assert run(b, "sparse") == 0
+    c = sparse_ints(14, 10)
+    c[4, :] = 0
+    assert run(c, "sparse") == 0
+
+
+@pytest.mark.parametrize("seed", [40, 41, 42])
+def test_walks_agree_at_order_30(seed):
+    # the decomposition leaves of the config-3 matrices are order 30
+    d = api.decompose(np.load(f"{GOLDEN_DIR}/c3_sparse01_n40_s{seed}.npy", allow_pickle=False))
+    for _, m in d.leaves:
+        if m.shape[0] < 20:
+            continue
+        b = m.astype(np.int64)
+        s = run(b, "sparse")
+        assert s == run(b, "int32")
+        try:
+            assert s == run(b, "fp64")
+        except _lib.SpmError as e:  # entries too large for the FP64 walk
+            assert e.status == _lib.SPM_ERANGE
+
+
+def test_config3_whole_matrix():
+    a = W.sparse01(40, 40).astype(np.int64)
+    assert run(a, "sparse") == 213647004
+    r = api.int_exact(a)  # the automatic choice takes the sparse walk here
+    assert r.exact == 213647004
