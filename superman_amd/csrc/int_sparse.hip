@@ -1,8 +1,9 @@
 // int_sparse.hip -- single-matrix exact Ryser kernel for sparse integer
 // matrices in compressed-column form (gfx950).  The per-chunk walk is in
 // walk_sparse.h; this file holds the grid-strided task loop over one
-// matrix's chunks and the block sum mod 2^128 into per-block records, which
-// int_dense.hip's reduce_i128_kernel sums.
+// matrix's chunks (each wave with its queue of deferred products in LDS),
+// the final flush of the queues and the block sum mod 2^128 into per-block
+// records, which int_dense.hip's reduce_i128_kernel sums.
 #include "walk_sparse.h"
 
 #ifndef SPM_SPARSE_ONLY_N
@@ -21,17 +22,19 @@ int_sparse_kernel(const int* __restrict__ blob,                // compressed col
                   uint64_t task_begin, uint64_t task_end, uint64_t num_chunks, int L, int accumulate)
 {
     __shared__ unsigned long long red[2 * (kDenseBlock / kWave)];
+    __shared__ int queue[(kDenseBlock / kWave) * kQCap * kQStride<N>];  // per-wave product queues
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = (uint64_t)blockIdx.x * (kDenseBlock / kWave) + (threadIdx.x / kWave);
     const uint64_t nwave = (uint64_t)gridDim.x * (kDenseBlock / kWave);
     unsigned __int128 acc = 0;
 
+    Queue Q{queue + (threadIdx.x / kWave) * kQCap * kQStride<N>, 0u};
+    const bool small = ((const ints::const_i32*)y0)[N] != 0;
     for (uint64_t task = task_begin + wave; task < task_end; task += nwave) {
         const uint64_t h = task * kWave + lane;
-        unsigned __int128 t = 0;
-        walk_sparse<N>(blob, y0, h, L, t);
-        if (h < num_chunks) acc += t;  // lanes past the end walk a dummy chunk
+        walk_sparse<N>(blob, y0, h, L, h < num_chunks, Q, lane, acc);  // lanes past the end walk a dummy chunk
     }
+    flush<N>(Q, small, lane, acc);
 
     // Block sum mod 2^128 (fixed order: wave butterfly, then waves in order).
     unsigned long long lo = (unsigned long long)acc, hi = (unsigned long long)(acc >> 64);

@@ -118,9 +118,9 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
 // with at most a quarter of their entries nonzero, from order 26 (below, a
 // permanent is too small a job for its per-chunk start), and -- when the
 // FP64 walk applies (small row sums) -- from order 34 (config-3 style 0/1
-// leaves of order 30 run equally fast on both; the whole order-40 matrix
-// 1.7x faster on the sparse walk).  With larger entries it replaces the
-// int32 walk (1.13x faster on order-30 leaves of the curtis54 example).
+// leaves of order 30 run equally fast on both; the whole order-40 0/1
+// matrix 1.3x faster on the sparse walk).  With larger entries it replaces
+// the int32 walk (2.1x faster on order-30 leaves of the curtis54 example).
 bool int_sparse_preferred(const int64_t* a, int n)
 {
     if (n < 26) return false;
