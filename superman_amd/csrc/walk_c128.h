@@ -112,7 +112,11 @@ __device__ __forceinline__ void stream_column(double (&x)[2 * N], const double* 
 // wait for a scalar load follows a product segment, so no load is waited
 // on right after it is issued (scalar loads complete out of order, so each
 // wait is for all of them).
-constexpr int kPartMax = 32;  // doubles (64 SGPRs)
+// SPM_C128_PART_MAX (sweeps only) overrides the part size.
+#ifndef SPM_C128_PART_MAX
+#define SPM_C128_PART_MAX 32
+#endif
+constexpr int kPartMax = SPM_C128_PART_MAX;  // doubles (64 SGPRs at 32)
 template <int N>
 constexpr int kPhases = (2 * N + kPartMax - 1) / kPartMax;
 template <int N>
