@@ -6,15 +6,17 @@
 // spSingleGPU.cuh:44-234): per Gray step only the rows of the flipped
 // column change, a count of zero rows is kept, and the product over all
 // rows is formed only when no row is zero -- here deferred through a
-// per-wave queue, so that 64 queued products are formed together.  As in the dense walks here,
-// every lane of a wavefront walks its own aligned chunk of 2^L steps and
-// all lanes flip the same column in lockstep, so a column's rows and values
-// are wave-uniform: a column comes in as 64-byte records (spm_kernels.h),
-// one scalar load each, prefetched one step ahead, and a row index selects
-// the register of y by VGPR indexing (the index is an SGPR), so y stays in
-// VGPRs.  A record's unused slots name the spare register y[N] with value 0,
-// so every record is applied without branches.  Arithmetic is exact (y = 2x
-// in int32, T mod 2^128, as walk_int.h).
+// per-wave queue, so that 64 queued products are formed together.
+//
+// As in the dense walks here, every lane of a wavefront walks its own
+// aligned chunk of 2^L steps and all lanes flip the same column in
+// lockstep, so a column's rows and values are wave-uniform: a column comes
+// in as 64-byte records (spm_kernels.h), one scalar load each, prefetched
+// one step ahead, and a row index selects the register of y by VGPR
+// indexing (the index is an SGPR), so y stays in VGPRs.  A record's slots
+// past its count are skipped by a uniform test (unused slots name the
+// spare register y[N] with value 0).  Arithmetic is exact (y = 2x in
+// int32, T mod 2^128, as walk_int.h).
 #ifndef SPM_WALK_SPARSE_H
 #define SPM_WALK_SPARSE_H
 
