@@ -76,3 +76,14 @@ def test_config3_whole_matrix():
     assert run(a, "sparse") == 213647004
     r = api.int_exact(a)  # the automatic choice takes the sparse walk here
     assert r.exact == 213647004
+
+
+@pytest.mark.parametrize("vmax,n", [(200, 12), (3000, 10), (1 << 26, 4)])
+def test_large_entries_every_product_form(vmax, n):
+    # row sums of 2^9, 2^13 and 2^26: products as int64 row pairs, exact up
+    # to the 2^127 bound
+    a = sparse_ints(n, 7 + n, per_row=3, vmax=vmax)
+    ref, st = O.int_exact(a)
+    assert st == 0
+    assert run(a, "sparse") == ref
+    assert run(a, "int32") == ref
