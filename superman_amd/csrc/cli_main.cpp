@@ -10,9 +10,12 @@
 // generation algorithms, which take the whole matrix -- then print
 //   Permanent: <value> - Computed in: <seconds> seconds.
 // Complex matrices (complex=true) go to gpuComputeComplex, which prints its
-// own line.  Keys: filename, algorithm, mode (single_gpu), calculation_precision
-// (dd | kahan), complex, binary, undirected, device_id; pid, repo_dir,
-// thread_count and gpu_num are accepted and ignored.
+// own line.  Keys as the reference's: filename, algorithm, mode (single_gpu
+// runs; cpu and the multi-GPU modes end with an error), calculation_precision
+// (dd | kahan; a number selects the reference's CPU-only arbitrary
+// precision, an error here), complex, binary, undirected, device_id; pid,
+// repo_dir, thread_count, gpu_num and the approximation's scaling keys are
+// read and have no effect.
 #include "superman/AlgorithmSelector.h"
 
 #include <chrono>
@@ -30,17 +33,42 @@
 
 namespace {
 
+int int_arg(const std::string& key, const std::string& v)
+{
+    try {
+        return std::stoi(v);
+    } catch (const std::exception&) {
+        throw std::runtime_error("An integer value should be provided to the " + key + " argument!\n");
+    }
+}
+
+unsigned unsigned_arg(const std::string& key, const std::string& v)
+{
+    try {
+        return (unsigned)std::stoul(v);
+    } catch (const std::exception&) {
+        throw std::runtime_error("An integer value should be provided to the " + key + " argument!\n");
+    }
+}
+
+// key=value settings as IO::readSettings (IO/IO.h:163-401) reads them:
+// unknown keys are reported and skipped, bad values end the run.  pid and
+// repo_dir, which the reference needs for its recompile wrapper, are
+// accepted and not required.
 Settings parse(int argc, char** argv)
 {
     Settings s;
     s.mode = SingleGPU;
+    bool have_file = false;
     for (int i = 1; i < argc; ++i) {
         const std::string kv = argv[i];
         const size_t eq = kv.find('=');
         if (eq == std::string::npos) throw std::runtime_error("Arguments are key=value pairs: " + kv + "\n");
         const std::string k = kv.substr(0, eq), v = kv.substr(eq + 1);
-        if (k == "filename") s.filename = v;
-        else if (k == "algorithm") {
+        if (k == "filename") {
+            s.filename = v;
+            have_file = true;
+        } else if (k == "algorithm") {
             if (v == "auto") s.algorithm = AlgorithmEnds;
             else if (v == "xregister_mshared") s.algorithm = XREGISTERMSHARED;
             else if (v == "xregister_mglobal") s.algorithm = XREGISTERMGLOBAL;
@@ -53,23 +81,44 @@ Settings parse(int argc, char** argv)
             else if (v == "approximation") s.algorithm = APPROXIMATION;
             else throw std::runtime_error("UNKNOWN ALGORITHM: " + v + '\n');
         } else if (k == "mode") {
-            if (v == "single_gpu") s.mode = SingleGPU;
-            else if (v == "cpu") s.mode = CPU;
+            if (v == "cpu") {
+                s.mode = CPU;
+                s.algorithm = AlgorithmEnds;
+            } else if (v == "single_gpu") s.mode = SingleGPU;
             else if (v == "multi_gpu") s.mode = MultiGPU;
             else if (v == "multi_gpu_mpi") s.mode = MultiGPUMPI;
             else throw std::runtime_error("UNKNOWN MODE: " + v + '\n');
         } else if (k == "calculation_precision") {
-            if (v == "dd") s.calculationPrecision = DD;
-            else if (v == "kahan") s.calculationPrecision = KAHAN;
-            else throw std::runtime_error("Arbitrary precision is only available on the CPU.\n");
+            if (v == "kahan") s.calculationPrecision = KAHAN;
+            else if (v == "dd") s.calculationPrecision = DD;
+            else {
+                try {
+                    s.arbitraryPrecision = (unsigned)std::stoul(v);
+                    s.calculationPrecision = ARBITRARY;
+                } catch (const std::exception&) {
+                    throw std::runtime_error(
+                        "Calculation precision can either be kahan or dd or an arbitrary precision given as an "
+                        "unsigned integer that will launch SUperman with arbitrarily selected mantissa (check IEEE "
+                        "754)!\n");
+                }
+            }
         } else if (k == "complex") s.complex = v == "true";
         else if (k == "binary") s.binary = v == "true";
         else if (k == "undirected") s.undirected = v == "true";
-        else if (k == "device_id") s.deviceID = std::stoi(v);
-        else if (k == "pid" || k == "repo_dir" || k == "thread_count" || k == "gpu_num") {
-        } else throw std::runtime_error("UNKNOWN ARGUMENT: " + k + '\n');
+        else if (k == "scaling") s.scaling = v == "true";
+        else if (k == "scaling_iteration_no") s.scalingIterationNo = unsigned_arg(k, v);
+        else if (k == "scale_into") s.scaleInto = unsigned_arg(k, v);
+        else if (k == "device_id") s.deviceID = int_arg(k, v);
+        else if (k == "thread_count") s.threadC = int_arg(k, v);
+        else if (k == "gpu_num") s.gpuNum = int_arg(k, v);
+        else if (k == "pid") s.PID = int_arg(k, v);
+        else if (k == "repo_dir") s.REPO_DIR = v;
+        else std::cout << "UNKNOWN ARGUMENT: " << k << " - skipping it." << std::endl;
     }
-    if (s.filename.empty()) throw std::runtime_error("A filename=<matrix.mtx> argument is required.\n");
+    if (!have_file)
+        throw std::runtime_error(
+            "You are required to provide the absolute path of the matrix file that you want to calculate the "
+            "matrix permanent for to the executable of this program!\n");
     return s;
 }
 
