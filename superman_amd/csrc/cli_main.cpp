@@ -227,6 +227,21 @@ int main(int argc, char** argv)
         for (Matrix<double>* b : blocks) delete b;
         const double secs =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // The leaf statistics line of main.cpp:122-126 (its counters are
+        // filled by DecomposePerman; the same decomposition gives them here).
+        {
+            spm_decomp* st = nullptr;
+            const int opts = per_component ? SPM_DECOMP_DEFAULT
+                                           : (SPM_DECOMP_DM | SPM_DECOMP_CODEGEN | SPM_DECOMP_COMPRESS);
+            if (spm_decompose(a.data(), n, opts, &st) == SPM_OK) {
+                int leaves = 0;
+                double avg_n = 0, avg_nnz = 0;
+                spm_decomp_stats(st, nullptr, &leaves, &avg_n, &avg_nnz, nullptr);
+                std::cout << "Submatrix count: " << leaves << " - Average N: " << avg_n
+                          << " - Average NNZ: " << avg_nnz << std::endl;
+                spm_decomp_destroy(st);
+            }
+        }
         if (std::isnan(perm))
             std::cout << "Permanent computation has overflowed. - Computed in: " << secs << " seconds." << std::endl;
         else
