@@ -122,46 +122,57 @@ Settings parse(int argc, char** argv)
     return s;
 }
 
-// Coordinate MatrixMarket (1-based) or plain coordinate file (0-based), real
-// or complex values; `pattern` files and binary=true read every entry as 1.
+// Coordinate MatrixMarket (1-based) or plain coordinate file (0-based), as
+// IO::readMatrix / IO::readComplex (IO/IO.h:403-596) read it, with their
+// three summary lines.  Real: binary=true (and, here, a `pattern` banner)
+// reads every entry as 1, zeros are dropped, undirected=true mirrors
+// off-diagonal entries; complex: every entry is (re, im), undirected=true
+// mirrors.
 template <class S>
 int read_matrix(const Settings& s, std::vector<S>& a)
 {
+    constexpr bool cplx = std::is_same<S, std::complex<double>>::value;
     std::ifstream f(s.filename);
     if (!f.is_open()) throw std::runtime_error("File could not be opened.\n");
     const bool mtx = s.filename.size() > 4 && s.filename.substr(s.filename.size() - 4) == ".mtx";
-    bool pattern = s.binary;
+    bool pattern = !cplx && s.binary;
     std::string line;
     while (f.peek() == '%') {
         std::getline(f, line);
-        if (line.find("pattern") != std::string::npos) pattern = true;
+        if (!cplx && line.find("pattern") != std::string::npos) pattern = true;
     }
     int n = 0, m = 0, lines = 0;
     f >> n >> m >> lines;
     if (n != m) throw std::runtime_error("Your matrix is not square, and therefore the permanent of it is trivially equal to 0.");
     a.assign((size_t)n * n, S(0));
+    long long nnz = 0;
     for (int e = 0; e < lines; ++e) {
         int i, j;
         f >> i >> j;
         S v = S(1);
-        if (!pattern) {
-            if constexpr (std::is_same<S, std::complex<double>>::value) {
-                double re, im;
-                f >> re >> im;
-                v = S(re, im);
-            } else {
-                f >> v;
-            }
+        if constexpr (cplx) {
+            double re, im;
+            f >> re >> im;
+            v = S(re, im);
+        } else if (!pattern) {
+            f >> v;
         }
-        if (v == S(0)) continue;
+        if (!cplx && v == S(0)) continue;
         if (mtx) {
             --i;
             --j;
         }
         if (i < 0 || j < 0 || i >= n || j >= n) throw std::runtime_error("Entry outside the matrix.\n");
         a[(size_t)i * n + j] = v;
-        if (s.undirected && i != j) a[(size_t)j * n + i] = v;
+        ++nnz;
+        if (s.undirected && (cplx || i != j)) {
+            a[(size_t)j * n + i] = v;
+            ++nnz;
+        }
     }
+    std::cout << "Number of rows/columns of matrix is: " << n << std::endl;
+    std::cout << "Total number of nonzeros is: " << nnz << std::endl;
+    std::cout << "Sparsity of the matrix is determined to be: " << double(nnz) / (double(n) * n) * 100 << std::endl;
     return n;
 }
 
