@@ -11,8 +11,8 @@ golden.json, which records for it
   * dd            -- the double-double oracle value (hi, lo);
   * exact         -- the exact-integer oracle value (decimal string);
   * known         -- a closed-form answer, where one exists.
-The reference binary only exists where the reference sources do; the GPU box
-reads golden.json and the .npy files only.
+The reference binary only exists where the reference sources do; the GPU
+tests read golden.json and the .npy files only.
 """
 from __future__ import annotations
 

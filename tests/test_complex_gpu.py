@@ -1,5 +1,5 @@
 """Dense complex128 Gray-code Ryser on the GPU vs the double-double oracle
-(through the C ABI).  Bar: relative error <= 1e-10 (north star), and the
+(through the C ABI).  Bar: relative error <= 1e-10, and the
 compensated mode inside its own stated error estimate."""
 import math
 

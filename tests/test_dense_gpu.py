@@ -1,6 +1,6 @@
 """Dense real FP64 Gray-code Ryser on the GPU vs the oracle (through the C ABI).
 
-Bar (north star): relative error <= 1e-10 against the double-double oracle;
+Bar: relative error <= 1e-10 against the double-double oracle;
 the compensated mode must also sit inside its own stated error estimate.
 At n = 40 (no oracle) the checks are size-independent properties."""
 import numpy as np
