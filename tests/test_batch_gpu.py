@@ -231,3 +231,22 @@ def test_rerun_after_a_failed_check_gives_the_same_results():
         assert rc == 0 and st.retried >= 5 and st.completed == 60
         for q, a, b in zip(reqs, first, again):
             assert _same(to_permanent(a, q[0]), to_permanent(b, q[0]))
+
+
+def test_many_batches_every_result_checked():
+    # hundreds of batches over two slots, every result against the one-call
+    # path: integers equal, floating point within the two error estimates
+    stream = W.serving_stream(3000, rate=4e4, seed=33, n_lo=8, n_hi=22)
+    reqs = [(q.kind, q.a, q.rid) for q in stream]
+    arr = np.array([q.arrival for q in stream])
+    with BatchEngine(slots=2, max_requests=32) as eng:
+        rc, res, _, st = eng.replay(reqs, arr)
+    assert rc == 0 and st.completed == 3000 and st.batches >= 100
+    calls = {"real": api.dense_f64, "complex": api.dense_c128, "int01": lambda a: api.int_exact(a.astype(np.int64))}
+    for (kind, a, _), r in zip(reqs, res):
+        b = to_permanent(r, kind)
+        p = calls[kind](a)
+        if kind == "int01":
+            assert b.exact == p.exact
+        else:
+            assert abs(b.value - p.value) <= b.err_est + p.err_est
