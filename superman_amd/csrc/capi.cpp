@@ -281,6 +281,13 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
     if (walk == SPM_INT_WALK_FP64 && !small) return SPM_ERANGE;
     const bool sp = walk == SPM_INT_WALK_SPARSE || (walk == SPM_INT_WALK_AUTO && int_sparse_preferred(a, n));
     const bool fp = !sp && (walk == SPM_INT_WALK_FP64 || (walk == SPM_INT_WALK_AUTO && small));
+    std::vector<int64_t> sorted;
+    if (sp) {  // sparsest columns first: flipped most often (ascending_col_sort)
+        sorted = ascending_col_sort(a, n);
+        a = sorted.data();
+        rc = int_prepare(a, n, y0, &lb);
+        if (rc != SPM_OK) return rc;
+    }
     int cus = 0;
     int st = device_cus(device, &cus);
     if (st != SPM_OK) return st;

@@ -74,6 +74,23 @@ void int_planes(const int64_t* a, int n, int* planes)
         }
 }
 
+// Columns in ascending order of their nonzero count, ties by index (the
+// order of IO::ascendingColSort, IO/IO.h:694-738): the permanent does not
+// change, and the Gray walk flips its low columns most often (column 0
+// every other step), so the sparse walk does the least work per step.
+std::vector<int64_t> ascending_col_sort(const int64_t* a, int n)
+{
+    std::vector<int> deg(n, 0), order(n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) deg[j] += a[(size_t)i * n + j] != 0 ? 1 : 0;
+    for (int j = 0; j < n; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return deg[x] < deg[y]; });
+    std::vector<int64_t> b((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) b[(size_t)i * n + j] = a[(size_t)i * n + order[j]];
+    return b;
+}
+
 // Compressed columns of +2A / -2A as the records of spm_kernels.h.
 void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
 {
@@ -114,20 +131,21 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
         }
 }
 
-// Where the sparse walk is the faster one (measured on MI355X): matrices
-// with at most a quarter of their entries nonzero, from order 26 (below, a
-// permanent is too small a job for its per-chunk start), and -- when the
-// FP64 walk applies (small row sums) -- from order 34 (config-3 style 0/1
-// leaves of order 30 run equally fast on both; the whole order-40 0/1
-// matrix 1.3x faster on the sparse walk).  With larger entries it replaces
-// the int32 walk (2.1x faster on order-30 leaves of the curtis54 example).
+// Where the sparse walk (columns in ascending_col_sort order) is the faster
+// one, measured on MI355X: matrices with at most a quarter of their entries
+// nonzero, from order 26 (below, a permanent is too small a job for its
+// per-chunk start), and -- when the FP64 walk applies (small row sums) --
+// from order 30 (config-3 style 0/1 leaves of order 30: 7% faster on
+// average; the whole order-40 0/1 matrix 1.5x; equal at order 28).  With
+// larger entries it replaces the int32 walk (2.3x faster on order-30
+// leaves of the curtis54 example).
 bool int_sparse_preferred(const int64_t* a, int n)
 {
     if (n < 26) return false;
     size_t nnz = 0;
     for (size_t e = 0; e < (size_t)n * n; ++e) nnz += a[e] != 0 ? 1 : 0;
     if (4 * nnz > (size_t)n * n) return false;
-    return int_small_ok(a, n) ? n >= 34 : true;
+    return int_small_ok(a, n) ? n >= 30 : true;
 }
 
 // True when every absolute row sum is below 2^int_small_bits(n), so that

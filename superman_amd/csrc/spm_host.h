@@ -53,6 +53,7 @@ void dense_prepare(const double* a, int n, int w, double* planes, double* x0);
 int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound);
 void int_planes(const int64_t* a, int n, int* planes);
 void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob);
+std::vector<int64_t> ascending_col_sort(const int64_t* a, int n);
 bool int_sparse_preferred(const int64_t* a, int n);
 bool int_small_ok(const int64_t* a, int n);
 void int_small_planes(const int64_t* a, int n, double* planes);
