@@ -241,7 +241,7 @@ def test_many_batches_every_result_checked():
     arr = np.array([q.arrival for q in stream])
     with BatchEngine(slots=2, max_requests=32) as eng:
         rc, res, _, st = eng.replay(reqs, arr)
-    assert rc == 0 and st.completed == 3000 and st.batches >= 100
+    assert rc == 0 and st.completed == 3000 and st.batches >= 3000 // 32  # at most 32 per batch
     calls = {"real": api.dense_f64, "complex": api.dense_c128, "int01": lambda a: api.int_exact(a.astype(np.int64))}
     for (kind, a, _), r in zip(reqs, res):
         b = to_permanent(r, kind)
