@@ -43,6 +43,9 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Seconds a finished batch may wait for a result's flag to reach host memory.
 constexpr double kPublishWait = 1.0;
+// Throughput the autotuner gives up for the fused kernel's latency.
+constexpr double kFusedMargin = 0.05;
+
 // Re-runs of a request whose result failed its checks (item records of
 // another batch in its reduction, an integer sum that is not a multiple of
 // 2^(n-1)) before it completes with SPM_EHIP.
@@ -871,7 +874,13 @@ extern "C" int spm_batch_autotune(spm_batch* b, int count, const spm_request* sa
     double win_s = 1e300;
     int st = time_it(cur, count, sample, &win_s);
     if (st != SPM_OK) return st;
-    // 1) kernel organisation and persistent grid size
+    // 1) kernel organisation and persistent grid size.  The fused kernel
+    // walks every kind in one shortest-first order, so a small request of
+    // any kind finishes early; the per-kind kernels run one after another on
+    // the slot's stream, so the last kind's requests wait for the others.
+    // Fused is kept unless per-kind is faster by more than kFusedMargin.
+    spm_batch_tuning best[2] = {win, win};
+    double best_s[2] = {1e300, 1e300};
     for (int fused = 0; fused <= 1; ++fused) {
         for (int wpc : {0, 4, 8, 12, 16}) {
             spm_batch_tuning t = win;
@@ -880,11 +889,16 @@ extern "C" int spm_batch_autotune(spm_batch* b, int count, const spm_request* sa
             double s = 0;
             st = candidate(t, count, sample, &s);
             if (st != SPM_OK) return st;
-            if (s < win_s) {
-                win_s = s;
-                win = t;
+            if (s < best_s[fused]) {
+                best_s[fused] = s;
+                best[fused] = t;
             }
         }
+    }
+    const int org = best_s[1] <= best_s[0] * (1.0 + kFusedMargin) ? 1 : 0;
+    if (best_s[org] < 1e300) {  // (the starting tuning is one of the candidates)
+        win_s = best_s[org];
+        win = best[org];
     }
     // 2) chunk size per kind, timed on that kind's share of the sample
     for (int k = 1; k < 4; ++k) {
