@@ -9,7 +9,8 @@
  * CUDA error (Algorithms/GPU/GPUHelpers.cuh:77-85); here every call returns a
  * status code and fills an extended `spm_result`, whose `re` field is exactly
  * what the reference's Result.permanent carries.  The C++ drop-in layer
- * (include/superman/*.h) maps these back onto Result / runtime_error.
+ * (the headers under include/superman/) maps these back onto Result /
+ * runtime_error.
  *
  * Matrices are row-major n x n (a[i*n + j] = a_ij), 1 <= n <= 63
  * (reference limit: CommonFiles/DecomposePerman.h:185-188).  Complex

@@ -879,8 +879,8 @@ extern "C" int spm_batch_autotune(spm_batch* b, int count, const spm_request* sa
     // any kind finishes early; the per-kind kernels run one after another on
     // the slot's stream, so the last kind's requests wait for the others.
     // Fused is kept unless per-kind is faster by more than kFusedMargin.
-    spm_batch_tuning best[2] = {win, win};
-    double best_s[2] = {1e300, 1e300};
+    spm_batch_tuning org_best[2] = {win, win};
+    double org_s[2] = {1e300, 1e300};
     for (int fused = 0; fused <= 1; ++fused) {
         for (int wpc : {0, 4, 8, 12, 16}) {
             spm_batch_tuning t = win;
@@ -889,16 +889,16 @@ extern "C" int spm_batch_autotune(spm_batch* b, int count, const spm_request* sa
             double s = 0;
             st = candidate(t, count, sample, &s);
             if (st != SPM_OK) return st;
-            if (s < best_s[fused]) {
-                best_s[fused] = s;
-                best[fused] = t;
+            if (s < org_s[fused]) {
+                org_s[fused] = s;
+                org_best[fused] = t;
             }
         }
     }
-    const int org = best_s[1] <= best_s[0] * (1.0 + kFusedMargin) ? 1 : 0;
-    if (best_s[org] < 1e300) {  // (the starting tuning is one of the candidates)
-        win_s = best_s[org];
-        win = best[org];
+    const int org = org_s[1] <= org_s[0] * (1.0 + kFusedMargin) ? 1 : 0;
+    if (org_s[org] < 1e300) {  // (the starting tuning is one of the candidates)
+        win_s = org_s[org];
+        win = org_best[org];
     }
     // 2) chunk size per kind, timed on that kind's share of the sample
     for (int k = 1; k < 4; ++k) {
