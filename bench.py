@@ -49,7 +49,7 @@ def host_cores() -> int:
 
 
 def cpu_baseline(a: np.ndarray, target_s: float = 12.0) -> dict:
-    from oracle import oracle as O  # the checker / CPU baseline only
+    from oracle import oracle as O  # CPU baseline timing only
     n = a.shape[0]
     cores = host_cores()
     count = 1 << 22

@@ -1,8 +1,8 @@
 """ctypes loader for the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker and as the CPU baseline.  The
-product package (superman_amd) never imports this module.
+cpu_baseline leg of bench.py, as the reference answer in tests and as the CPU
+baseline.  The product package (superman_amd) never imports this module.
 
 Every function takes a numpy array and returns Python numbers; see
 ryser_oracle.c for the reference file:line each C function restates.

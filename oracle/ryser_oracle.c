@@ -1,9 +1,9 @@
 /*
  * ryser_oracle.c -- CPU oracle for the exact-permanent hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker and the CPU baseline:
- * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
- * it (through oracle/oracle.py).  The product library (superman_amd/csrc)
+ * TEST INFRASTRUCTURE ONLY.  This file gives the reference answers for tests
+ * and the CPU baseline: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it (through oracle/oracle.py).  The product library (superman_amd/csrc)
  * never links, loads or calls anything here.
  *
  * Contents
