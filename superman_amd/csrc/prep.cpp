@@ -107,6 +107,7 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
         extra += k > kSparseSlots ? (k - 1) / kSparseSlots : 0;
     }
     const int nrec = 2 * n + 2 * extra;
+    const int bank = sparse_bank_rows(n);
     blob.assign((size_t)nrec * kSparseRec, 0);
     int next_free = 2 * n;
     for (int j = 0; j < n; ++j)
@@ -115,12 +116,15 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
             int rec = 2 * j + plane;
             for (int e0 = 0;; e0 += kSparseSlots) {
                 int* r = blob.data() + (size_t)rec * kSparseRec;
-                for (int u = 0; u < kSparseSlots; ++u) {
-                    const int e = e0 + u;
-                    r[1 + u] = e < k ? rows[j][e] : n;
-                    r[1 + kSparseSlots + u] = e < k ? (plane ? -vals[j][e] : vals[j][e]) : 0;
+                int ca = 0, cb = 0;  // bank-a slots from 0 up, bank-b slots from 6 down
+                for (int e = e0; e < std::min(k, e0 + kSparseSlots); ++e) {
+                    const int row = rows[j][e];
+                    const int v = plane ? -vals[j][e] : vals[j][e];
+                    const int s = row < bank ? ca++ : kSparseSlots - 1 - cb++;
+                    r[1 + s] = row < bank ? row : row - bank;
+                    r[1 + kSparseSlots + s] = v;
                 }
-                r[kSparseRec - 1] = std::min(kSparseSlots, k - e0);
+                r[kSparseRec - 1] = ca + (cb << 16);
                 if (e0 + kSparseSlots >= k) {
                     r[0] = -1;
                     break;
@@ -134,18 +138,17 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
 // Where the sparse walk (columns in ascending_col_sort order) is the faster
 // one, measured on MI355X: matrices with at most a quarter of their entries
 // nonzero, from order 26 (below, a permanent is too small a job for its
-// per-chunk start), and -- when the FP64 walk applies (small row sums) --
-// from order 30 (config-3 style 0/1 leaves of order 30: 7% faster on
-// average; the whole order-40 0/1 matrix 1.5x; equal at order 28).  With
-// larger entries it replaces the int32 walk (2.3x faster on order-30
-// leaves of the curtis54 example).
+// per-chunk start).  Against the FP64 walk on 0/1 matrices: equal at order
+// 24 (4 entries per row), 1.5x at order 28, 1.2x on average on the config-3
+// leaves of order 30 and 2.5x on the whole order-40 matrix; with larger
+// entries it replaces the int32 walk (2.3x faster on order-30 leaves of
+// the curtis54 example).
 bool int_sparse_preferred(const int64_t* a, int n)
 {
     if (n < 26) return false;
     size_t nnz = 0;
     for (size_t e = 0; e < (size_t)n * n; ++e) nnz += a[e] != 0 ? 1 : 0;
-    if (4 * nnz > (size_t)n * n) return false;
-    return int_small_ok(a, n) ? n >= 30 : true;
+    return 4 * nnz <= (size_t)n * n;
 }
 
 // True when every absolute row sum is below 2^int_small_bits(n), so that

@@ -13,10 +13,14 @@
 // lockstep, so a column's rows and values are wave-uniform: a column comes
 // in as 64-byte records (spm_kernels.h), one scalar load each, prefetched
 // one step ahead, and a row index selects the register of y by VGPR
-// indexing (the index is an SGPR), so y stays in VGPRs.  A record's slots
-// past its count are skipped by a uniform test (unused slots name the
-// spare register y[N] with value 0).  Arithmetic is exact (y = 2x in
-// int32, T mod 2^128, as walk_int.h).
+// indexing (the index is an SGPR), so y stays in VGPRs.  An indexed access
+// reaches within one register tuple of at most 32 registers, so past order
+// 32 the rows are two arrays ("banks", split at sparse_bank_rows), each
+// indexed on its own; a record lists its bank-a entries from slot 0 up and
+// its bank-b entries from slot 6 down, so the bank of every slot is known
+// at compile time.  The slots in use are found by uniform tests that stop
+// at the first empty slot.  Arithmetic is exact (y = 2x in int32, T mod
+// 2^128, as walk_int.h).
 #ifndef SPM_WALK_SPARSE_H
 #define SPM_WALK_SPARSE_H
 
@@ -46,41 +50,85 @@ __device__ __forceinline__ void load_rec(Rec& r, const int* recs, int idx)
 // The record of column j in plane p (0: +2A, 1: -2A).
 __device__ __forceinline__ int rec_index(int j, int p) { return 2 * j + p; }
 
-// y[row] += value for the slots in use of one record (a uniform count:
-// the skipped slots cost a scalar branch, not the vector update), keeping
-// Z = number of zero rows (the spare y[N] is never zero).  MODE 0: values
-// as stored; 1: negated on lanes with m = -1 (per-lane direction); 2: kept
-// on lanes with m = -1 only (chunk-start membership).
-template <int N, int MODE>
-__device__ __forceinline__ void apply_slots(int (&y)[N + 1], int& Z, const Rec& r, int m)
+// Row sums of the walk: rows below sparse_bank_rows(N) in ya, the rest in
+// yb, two separate arrays (one aggregate would not be kept in registers).  Only
+// compile-time indexes reach both; run-time (wave-uniform) row indexes come
+// with their bank from the record layout.
+template <int N>
+constexpr int kRowsA = sparse_bank_rows(N);
+template <int N>
+constexpr int kRowsB = N - kRowsA<N>;
+template <int N>
+using BankA = int[kRowsA<N>];
+template <int N>
+using BankB = int[kRowsB<N> > 0 ? kRowsB<N> : 1];
+
+// y[row] += value (MODE 0: as stored; 1: negated on lanes with m = -1,
+// the per-lane direction; 2: kept on lanes with m = -1 only, chunk-start
+// membership), keeping Z = number of zero rows.
+template <int MODE, int R>
+__device__ __forceinline__ void update_row(int (&y)[R], int& Z, int row, int v, int m)
 {
-    const int used = r.v[kSparseRec - 1];
-#pragma unroll
-    for (int u = 0; u < kSparseSlots; ++u) {
-        if (u < used) {
-            const int row = r.v[1 + u];
-            int v = r.v[1 + kSparseSlots + u];
-            if constexpr (MODE == 1) v = (v ^ m) - m;
-            else if constexpr (MODE == 2) v &= m;
-            const int old = y[row];
-            const int now = old + v;
-            y[row] = now;
-            Z += (now == 0) - (old == 0);
+    if constexpr (MODE == 1) v = (v ^ m) - m;
+    else if constexpr (MODE == 2) v &= m;
+    const int old = y[row];
+    const int now = old + v;
+    y[row] = now;
+    Z += (now == 0) - (old == 0);
+}
+
+// Slots U, U+1, ... (bank a) while U < used; nested, so a record costs one
+// scalar test per entry plus the first failing one.
+template <int U, int MODE, int R>
+__device__ __forceinline__ void slots_up(int (&y)[R], int& Z, const Rec& r, int used, int m)
+{
+    if constexpr (U < kSparseSlots) {
+        if (U < used) {
+            update_row<MODE>(y, Z, r.v[1 + U], r.v[1 + kSparseSlots + U], m);
+            slots_up<U + 1, MODE>(y, Z, r, used, m);
         }
+    }
+}
+
+// Slots 6-U, 5-U, ... (bank b) while U < used.
+template <int U, int MODE, int R>
+__device__ __forceinline__ void slots_down(int (&y)[R], int& Z, const Rec& r, int used, int m)
+{
+    if constexpr (U < kSparseSlots) {
+        if (U < used) {
+            constexpr int s = kSparseSlots - 1 - U;
+            update_row<MODE>(y, Z, r.v[1 + s], r.v[1 + kSparseSlots + s], m);
+            slots_down<U + 1, MODE>(y, Z, r, used, m);
+        }
+    }
+}
+
+// The entries of one record: the count word is the bank-a count, plus the
+// bank-b count times 2^16 when N > 32.
+template <int N, int MODE>
+__device__ __forceinline__ void apply_slots(BankA<N>& ya, BankB<N>& yb, int& Z, const Rec& r, int m)
+{
+    const int cnt = r.v[kSparseRec - 1];
+    if constexpr (kRowsB<N> == 0) {
+        slots_up<0, MODE>(ya, Z, r, cnt, m);
+    } else {
+        slots_up<0, MODE>(ya, Z, r, cnt & 0xffff, m);
+        slots_down<0, MODE>(yb, Z, r, cnt >> 16, m);
     }
 }
 
 // A whole column from its first record (already loaded); longer columns
 // continue in further records, loaded here (rare for sparse inputs).
 template <int N, int MODE>
-__device__ __forceinline__ void apply_column(int (&y)[N + 1], int& Z, const Rec& first, const int* recs, int m)
+__device__ __forceinline__ void apply_column(BankA<N>& ya, BankB<N>& yb, int& Z, const Rec& first, const int* recs,
+                                             int m)
 {
-    apply_slots<N, MODE>(y, Z, first, m);
+    apply_slots<N, MODE>(ya, yb, Z, first, m);
     int nx = first.v[0];
     while (nx >= 0) {
         Rec r;
         load_rec(r, recs, nx);
-        apply_slots<N, MODE>(y, Z, r, m);
+        apply_slots<N, MODE>(ya, yb, Z, r, m);
         nx = r.v[0];
     }
 }
@@ -155,7 +203,7 @@ __device__ __forceinline__ void flush(Queue& Q, bool small, uint32_t lane, unsig
 
 // Queues +/- prod_r y_r for the active lanes with no zero row.
 template <int N>
-__device__ __forceinline__ void add_term(const int (&y)[N + 1], int Z, bool negative, bool small, bool active,
+__device__ __forceinline__ void add_term(const BankA<N>& ya, const BankB<N>& yb, int Z, bool negative, bool small, bool active,
                                          Queue& Q, uint32_t lane, unsigned __int128& acc)
 {
     const uint64_t need = __builtin_amdgcn_ballot_w64(active && Z == 0);
@@ -167,7 +215,9 @@ __device__ __forceinline__ void add_term(const int (&y)[N + 1], int Z, bool nega
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
             int* e = Q.q + (Q.count + below) * kQStride<N>;
 #pragma unroll
-            for (int r = 0; r < N; ++r) e[r] = y[r];
+            for (int r = 0; r < kRowsA<N>; ++r) e[r] = ya[r];
+#pragma unroll
+            for (int r = 0; r < kRowsB<N>; ++r) e[kRowsA<N> + r] = yb[r];
             e[N] = negative ? 1 : 0;
         }
         Q.count += cnt;
@@ -183,12 +233,14 @@ template <int N>
 __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const int* __restrict__ y0, uint64_t h,
                                             int L, bool active, Queue& Q, uint32_t lane, unsigned __int128& acc)
 {
-    int y[N + 1];
+    BankA<N> ya;
+    BankB<N> yb;
     {
         const const_i32* __restrict__ ys = step_ptr(y0);
 #pragma unroll
-        for (int r = 0; r < N; ++r) y[r] = ys[r];
-        y[N] = 1;
+        for (int r = 0; r < kRowsA<N>; ++r) ya[r] = ys[r];
+#pragma unroll
+        for (int r = 0; r < kRowsB<N>; ++r) yb[r] = ys[kRowsA<N> + r];
     }
     const bool small = ((const const_i32*)y0)[N] != 0;
     int Z = 0;
@@ -197,21 +249,23 @@ __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const 
     for (int j = (L > 0 ? L - 1 : 0); j < N - 1; ++j) {
         Rec r;
         load_rec(r, recs, rec_index(j, 0));
-        apply_column<N, 2>(y, Z, r, recs, -(int)((g0 >> j) & 1u));
+        apply_column<N, 2>(ya, yb, Z, r, recs, -(int)((g0 >> j) & 1u));
     }
     Z = 0;
 #pragma unroll
-    for (int r = 0; r < N; ++r) Z += y[r] == 0 ? 1 : 0;
+    for (int r = 0; r < kRowsA<N>; ++r) Z += ya[r] == 0 ? 1 : 0;
+#pragma unroll
+    for (int r = 0; r < kRowsB<N>; ++r) Z += yb[r] == 0 ? 1 : 0;
     const int lane_mask = -(int)(h & 1u);  // direction of column L-1: -1 = remove
 
     if constexpr (N == 1) {
-        add_term<N>(y, Z, false, small, active, Q, lane, acc);
+        add_term<N>(ya, yb, Z, false, small, active, Q, lane, acc);
     } else if constexpr (N == 2) {  // L = 1
-        add_term<N>(y, Z, false, small, active, Q, lane, acc);
+        add_term<N>(ya, yb, Z, false, small, active, Q, lane, acc);
         Rec r;
         load_rec(r, recs, rec_index(0, 0));
-        apply_column<N, 1>(y, Z, r, recs, lane_mask);
-        add_term<N>(y, Z, true, small, active, Q, lane, acc);
+        apply_column<N, 1>(ya, yb, Z, r, recs, lane_mask);
+        add_term<N>(ya, yb, Z, true, small, active, Q, lane, acc);
     } else {
         const uint32_t nblk = 1u << (L - 1);
         const uint32_t half = nblk >> 1;
@@ -220,24 +274,24 @@ __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const 
             return rec_index(tz + 1, (int)((m >> (tz + 1)) & 1u));
         };
         Rec ra, rb;
-        add_term<N>(y, Z, false, small, active, Q, lane, acc);  // k = 0
+        add_term<N>(ya, yb, Z, false, small, active, Q, lane, acc);  // k = 0
         load_rec(rb, recs, rec_index(0, 0));
-        apply_column<N, 0>(y, Z, rb, recs, 0);  // k = 1: +column 0
+        apply_column<N, 0>(ya, yb, Z, rb, recs, 0);  // k = 1: +column 0
         load_rec(ra, recs, a_rec(1));
         fence();
-        add_term<N>(y, Z, true, small, active, Q, lane, acc);
+        add_term<N>(ya, yb, Z, true, small, active, Q, lane, acc);
         fence();
         auto block = [&](uint32_t m, auto SPECIAL) __attribute__((always_inline)) {
-            if constexpr (decltype(SPECIAL)::value) apply_column<N, 1>(y, Z, ra, recs, lane_mask);
-            else apply_column<N, 0>(y, Z, ra, recs, 0);
+            if constexpr (decltype(SPECIAL)::value) apply_column<N, 1>(ya, yb, Z, ra, recs, lane_mask);
+            else apply_column<N, 0>(ya, yb, Z, ra, recs, 0);
             load_rec(rb, recs, rec_index(0, (int)(m & 1u)));  // step 2m+1: column 0
             fence();
-            add_term<N>(y, Z, false, small, active, Q, lane, acc);
+            add_term<N>(ya, yb, Z, false, small, active, Q, lane, acc);
             fence();
-            apply_column<N, 0>(y, Z, rb, recs, 0);
+            apply_column<N, 0>(ya, yb, Z, rb, recs, 0);
             load_rec(ra, recs, a_rec(m + 1));  // m+1 = nblk: column L <= N-1, a valid record
             fence();
-            add_term<N>(y, Z, true, small, active, Q, lane, acc);
+            add_term<N>(ya, yb, Z, true, small, active, Q, lane, acc);
             fence();
         };
 #pragma nounroll

@@ -1,7 +1,8 @@
 """Exact sparse walk over compressed columns (SPM_INT_WALK_SPARSE): bit-exact
 against the int128 oracle and against the two dense integer walks, on 0/1
-and small-integer sparse matrices, zero rows and columns, and the config-3
-generator at full size."""
+and small-integer sparse matrices, zero rows and columns, the config-3
+generator at full size, and closed forms on both sides of the register-bank
+split at order 32."""
 import numpy as np
 import pytest
 
@@ -87,3 +88,41 @@ def test_large_entries_every_product_form(vmax, n):
     assert st == 0
     assert run(a, "sparse") == ref
     assert run(a, "int32") == ref
+
+
+def fib(k):
+    x, y = 0, 1
+    for _ in range(k):
+        x, y = y, x + y
+    return x
+
+
+@pytest.mark.parametrize("n", [31, 32, 33, 34, 36, 41])
+def test_tridiagonal_fibonacci(n):
+    # perm of the 0/1 tridiagonal matrix: p(n) = p(n-1) + p(n-2), p(1) = 1,
+    # p(2) = 2, so F(n+1).  Past order 32 the rows sit in two register
+    # banks and the columns at the split have entries in both.
+    a = np.eye(n, dtype=np.int64) + np.eye(n, k=1, dtype=np.int64) + np.eye(n, k=-1, dtype=np.int64)
+    assert run(a, "sparse") == fib(n + 1)
+
+
+@pytest.mark.parametrize("n", [33, 35])
+def test_identity_plus_cycle(n):
+    # only the identity and the n-cycle avoid the zeros of I + P
+    a = np.eye(n, dtype=np.int64) + np.roll(np.eye(n, dtype=np.int64), 1, axis=1)
+    assert run(a, "sparse") == 2
+
+
+@pytest.mark.parametrize("n", [33, 34, 36])
+def test_both_banks_long_columns(n):
+    # signed entries, two 12-entry columns (continuation records with rows in
+    # both banks); the int32 dense walk and a row-permuted copy must agree
+    a = sparse_ints(n, 500 + n, per_row=4, vmax=3, signed=True)
+    rng = np.random.default_rng(n)
+    for j in (0, n - 1):
+        rows = rng.choice(n, 12, replace=False)
+        a[rows, j] = rng.integers(1, 4, 12)
+    s = run(a, "sparse")
+    assert s == run(a, "int32")
+    p = np.random.default_rng(7).permutation(n)
+    assert run(a[p], "sparse") == s
