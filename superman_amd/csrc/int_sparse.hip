@@ -14,21 +14,26 @@ namespace spm {
 
 namespace sparse {
 
+// Orders whose queues are sized for 4 blocks per CU (kQCap) ask the
+// register allocator for the matching 4 waves per SIMD.
 template <int N>
-__global__ void __launch_bounds__(kDenseBlock)
+constexpr int kSparseWaves = kQCap<N> < 64 ? 4 : 1;
+
+template <int N>
+__global__ void __launch_bounds__(kDenseBlock) __attribute__((amdgpu_waves_per_eu(kSparseWaves<N>)))
 int_sparse_kernel(const int* __restrict__ blob,                // compressed columns (walk_sparse.h)
                   const int* __restrict__ y0,                  // y(empty set), N values
                   unsigned long long* __restrict__ partial,    // [gridDim.x][2]: T mod 2^128 (lo, hi)
                   uint64_t task_begin, uint64_t task_end, uint64_t num_chunks, int L, int accumulate)
 {
     __shared__ unsigned long long red[2 * (kDenseBlock / kWave)];
-    __shared__ int queue[(kDenseBlock / kWave) * kQCap * kQStride<N>];  // per-wave product queues
+    __shared__ int queue[(kDenseBlock / kWave) * kQCap<N> * kQStride<N>];  // per-wave product queues
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = (uint64_t)blockIdx.x * (kDenseBlock / kWave) + (threadIdx.x / kWave);
     const uint64_t nwave = (uint64_t)gridDim.x * (kDenseBlock / kWave);
     unsigned __int128 acc = 0;
 
-    Queue Q{queue + (threadIdx.x / kWave) * kQCap * kQStride<N>, 0u};
+    Queue Q{queue + (threadIdx.x / kWave) * kQCap<N> * kQStride<N>, 0u};
     const bool small = ((const ints::const_i32*)y0)[N] != 0;
     for (uint64_t task = task_begin + wave; task < task_end; task += nwave) {
         const uint64_t h = task * kWave + lane;

@@ -51,9 +51,9 @@ __device__ __forceinline__ void load_rec(Rec& r, const int* recs, int idx)
 __device__ __forceinline__ int rec_index(int j, int p) { return 2 * j + p; }
 
 // Row sums of the walk: rows below sparse_bank_rows(N) in ya, the rest in
-// yb, two separate arrays (one aggregate would not be kept in registers).  Only
-// compile-time indexes reach both; run-time (wave-uniform) row indexes come
-// with their bank from the record layout.
+// yb, two separate arrays (one aggregate would not be kept in registers).
+// Only compile-time indexes reach both; run-time (wave-uniform) row indexes
+// come with their bank from the record layout.
 template <int N>
 constexpr int kRowsA = sparse_bank_rows(N);
 template <int N>
@@ -173,10 +173,14 @@ __device__ __forceinline__ unsigned __int128 row_product(const int (&y)[N], bool
 // its row sums and sign into the wave's LDS queue, and when the queue is
 // full every lane forms one queued product.  Entries are kQStride ints (N
 // rows, the sign), an odd stride so that 64 lanes reading one entry each
-// touch distinct banks.
-constexpr uint32_t kQCap = 64;
+// touch distinct banks.  A queue holds 64 entries, except at orders 39..44,
+// where 64 would hold the kernel to 3 blocks of 4 waves per CU although
+// its registers allow 4: there it holds what fits in a sixteenth of the
+// CU's 160 KiB LDS less the block's reduction records.
 template <int N>
 constexpr int kQStride = (N + 1) | 1;
+template <int N>
+constexpr uint32_t kQCap = (N >= 39 && N <= 44) ? (uint32_t)((160 * 1024 / 4 - 512) / (4 * 4 * kQStride<N>)) : 64u;
 
 struct Queue {
     int* q;          // this wave's kQCap x kQStride ints
@@ -186,6 +190,7 @@ struct Queue {
 template <int N>
 __device__ __forceinline__ void flush(Queue& Q, bool small, uint32_t lane, unsigned __int128& acc)
 {
+    static_assert(kQCap<N> >= 1 && kQCap<N> <= 64, "one queued product per lane");
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane < Q.count) {
@@ -201,34 +206,59 @@ __device__ __forceinline__ void flush(Queue& Q, bool small, uint32_t lane, unsig
     Q.count = 0;
 }
 
+// Copies this lane's row sums and sign into queue entry `slot`.
+template <int N>
+__device__ __forceinline__ void enqueue(const BankA<N>& ya, const BankB<N>& yb, bool negative, Queue& Q,
+                                        uint32_t slot)
+{
+    int* e = Q.q + slot * kQStride<N>;
+#pragma unroll
+    for (int r = 0; r < kRowsA<N>; ++r) e[r] = ya[r];
+#pragma unroll
+    for (int r = 0; r < kRowsB<N>; ++r) e[kRowsA<N> + r] = yb[r];
+    e[N] = negative ? 1 : 0;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 // Queues +/- prod_r y_r for the active lanes with no zero row.
 template <int N>
-__device__ __forceinline__ void add_term(const BankA<N>& ya, const BankB<N>& yb, int Z, bool negative, bool small, bool active,
-                                         Queue& Q, uint32_t lane, unsigned __int128& acc)
+__device__ __forceinline__ void add_term(const BankA<N>& ya, const BankB<N>& yb, int Z, bool negative, bool small,
+                                         bool active, Queue& Q, uint32_t lane, unsigned __int128& acc)
 {
-    const uint64_t need = __builtin_amdgcn_ballot_w64(active && Z == 0);
-    if (__builtin_expect(need != 0, 0)) {
-        const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
-        if (Q.count + cnt > kQCap) flush<N>(Q, small, lane, acc);
-        if (active && Z == 0) {
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
-            int* e = Q.q + (Q.count + below) * kQStride<N>;
-#pragma unroll
-            for (int r = 0; r < kRowsA<N>; ++r) e[r] = ya[r];
-#pragma unroll
-            for (int r = 0; r < kRowsB<N>; ++r) e[kRowsA<N> + r] = yb[r];
-            e[N] = negative ? 1 : 0;
+    uint64_t need = __builtin_amdgcn_ballot_w64(active && Z == 0);
+    if constexpr (kQCap<N> == 64) {
+        if (__builtin_expect(need != 0, 0)) {
+            const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
+            if (Q.count + cnt > kQCap<N>) flush<N>(Q, small, lane, acc);
+            if (active && Z == 0) enqueue<N>(ya, yb, negative, Q, Q.count + lanes_below(need));
+            Q.count += cnt;
         }
-        Q.count += cnt;
+    } else {
+        // A ballot may name more lanes than a queue holds: take the lowest
+        // lanes that fit, flush, and go on with the rest.
+        while (__builtin_expect(need != 0, 0)) {
+            if (Q.count != 0 && Q.count + (uint32_t)__builtin_popcountll(need) > kQCap<N>)
+                flush<N>(Q, small, lane, acc);
+            uint64_t take = need;
+            while ((uint32_t)__builtin_popcountll(take) > kQCap<N> - Q.count)
+                take &= ~(1ull << (63 - __builtin_clzll(take)));
+            if ((take >> lane) & 1ull) enqueue<N>(ya, yb, negative, Q, Q.count + lanes_below(take));
+            Q.count += (uint32_t)__builtin_popcountll(take);
+            need &= ~take;
+            if (need != 0) flush<N>(Q, small, lane, acc);
+        }
     }
 }
 
 // One lane's chunk: the 2^L Gray steps starting at k = h 2^L, its nonzero
 // terms queued (inactive lanes -- past the matrix's last chunk -- queue
-// nothing); flushed products go into acc (mod 2^128).  Steps as in walk_int.h: k = 2m flips column ctz(m)+1
-// in a uniform direction, k = 2m+1 flips column 0, block nblk/2 flips
-// column L-1 in a per-lane direction.
+// nothing); flushed products go into acc (mod 2^128).  Steps as in
+// walk_int.h: k = 2m flips column ctz(m)+1 in a uniform direction, k = 2m+1
+// flips column 0, block nblk/2 flips column L-1 in a per-lane direction.
 template <int N>
 __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const int* __restrict__ y0, uint64_t h,
                                             int L, bool active, Queue& Q, uint32_t lane, unsigned __int128& acc)

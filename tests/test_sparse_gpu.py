@@ -126,3 +126,13 @@ def test_both_banks_long_columns(n):
     assert s == run(a, "int32")
     p = np.random.default_rng(7).permutation(n)
     assert run(a[p], "sparse") == s
+
+
+@pytest.mark.parametrize("n", [30, 39])
+def test_every_term_nonzero(n):
+    # I + 2P, P the n-cycle: every row sum is +-1 +- 2, never zero, so every
+    # lane queues a product at every step (at order 39 a queue holds fewer
+    # than 64 entries and one ballot overfills it); only the identity and
+    # the full cycle survive: 1 + 2^n
+    a = np.eye(n, dtype=np.int64) + 2 * np.roll(np.eye(n, dtype=np.int64), 1, axis=1)
+    assert run(a, "sparse") == 1 + 2**n
