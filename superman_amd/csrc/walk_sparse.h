@@ -179,8 +179,14 @@ __device__ __forceinline__ unsigned __int128 row_product(const int (&y)[N], bool
 // CU's 160 KiB LDS less the block's reduction records.
 template <int N>
 constexpr int kQStride = (N + 1) | 1;
+constexpr int kCuLdsBytes = 160 * 1024;
+constexpr int kQBlocksPerCu = 4;
+constexpr int kQBlockReserve = 512;  // the block's reduction records, rounded to the allocation unit
 template <int N>
-constexpr uint32_t kQCap = (N >= 39 && N <= 44) ? (uint32_t)((160 * 1024 / 4 - 512) / (4 * 4 * kQStride<N>)) : 64u;
+constexpr uint32_t kQCap =
+    (N >= 39 && N <= 44)
+        ? (uint32_t)((kCuLdsBytes / kQBlocksPerCu - kQBlockReserve) / ((kDenseBlock / kWave) * kQStride<N> * 4))
+        : 64u;
 
 struct Queue {
     int* q;          // this wave's kQCap x kQStride ints
