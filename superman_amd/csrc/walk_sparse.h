@@ -187,6 +187,9 @@ constexpr uint32_t kQCap =
     (N >= 39 && N <= 44)
         ? (uint32_t)((kCuLdsBytes / kQBlocksPerCu - kQBlockReserve) / ((kDenseBlock / kWave) * kQStride<N> * 4))
         : 64u;
+template <int N>
+constexpr bool kQFitsFourBlocks =
+    (kDenseBlock / kWave) * (int)kQCap<N> * kQStride<N> * 4 + kQBlockReserve <= kCuLdsBytes / kQBlocksPerCu;
 
 struct Queue {
     int* q;          // this wave's kQCap x kQStride ints
@@ -197,6 +200,7 @@ template <int N>
 __device__ __forceinline__ void flush(Queue& Q, bool small, uint32_t lane, unsigned __int128& acc)
 {
     static_assert(kQCap<N> >= 1 && kQCap<N> <= 64, "one queued product per lane");
+    static_assert(kQCap<N> == 64 || kQFitsFourBlocks<N>, "capped queues are sized for 4 blocks per CU");
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane < Q.count) {
