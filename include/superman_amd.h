@@ -292,7 +292,9 @@ int spm_batch_get_tuning(const spm_batch* b, spm_batch_tuning* t);
 int spm_batch_set_tuning(spm_batch* b, const spm_batch_tuning* t);
 /* Times `sample` (all submitted at once, best of `reps`) under candidate
  * tunings -- fused or per-kind kernels, persistent grid size, per-kind chunk
- * size -- and keeps the fastest.  The engine must be idle. */
+ * size -- and keeps the fastest, except where latency decides: the fused
+ * kernel and the smallest chunk are kept while within 5 % of the fastest
+ * time.  The engine must be idle. */
 int spm_batch_autotune(spm_batch* b, int count, const spm_request* sample, int reps, spm_batch_tuning* best,
                        double* best_seconds);
 /* Computes `count` permanents (as many batches as admission needs) and
