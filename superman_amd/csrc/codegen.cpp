@@ -109,6 +109,14 @@ namespace {
 
 constexpr int kGenU = 3;  // log2 of the steps per generated block
 
+// Orders whose kernels are asked for 5 waves per SIMD: from order 29 the
+// natural register allocation (100-127 VGPRs) leaves 4; at 96 VGPRs the
+// compiler spills only around the term product and the final reduction
+// (config-3 order-40 pattern: 155 -> 141 ms).  Orders up to 28 reach 5
+// without it; past 40 more rows would spill into the steps.
+constexpr int kGenWaves5Min = 29;
+constexpr int kGenWaves5Max = 40;
+
 std::string lit(int64_t v)
 {
     return std::to_string(v);  // integers are exact in decimal
@@ -168,7 +176,9 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
     }
     std::ostringstream o;
     o << "// generated: SpaRyser kernel for one " << n << "x" << n << " integer pattern\n";
-    o << "extern \"C\" __global__ void __launch_bounds__(256)\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(256)";
+    if (n >= kGenWaves5Min && n <= kGenWaves5Max) o << " __attribute__((amdgpu_waves_per_eu(5)))";
+    o << "\n";
     o << name << "(unsigned long long* __restrict__ partial, unsigned long long task_begin,\n"
       << "    unsigned long long task_end, unsigned long long num_chunks, int L, int accumulate)\n{\n";
     o << "    __shared__ unsigned long long red[8];\n"
