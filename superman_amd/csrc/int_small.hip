@@ -11,8 +11,14 @@
 
 namespace spm {
 
+// Orders 37..40 ask the register allocator for 4 waves per SIMD: their
+// natural allocation (128-133 VGPRs) leaves 3, and at 128 the spills fall in
+// the task loop (once per chunk), not in the steps.
 template <int N>
-__global__ void __launch_bounds__(kDenseBlock)
+constexpr int kIntFpWaves = (N >= 37 && N <= 40) ? 4 : 1;
+
+template <int N>
+__global__ void __launch_bounds__(kDenseBlock) __attribute__((amdgpu_waves_per_eu(kIntFpWaves<N>)))
 int_small_kernel(const double* __restrict__ M,            // [2][N][N]: +2A, -2A column-major
                  const double* __restrict__ y0,           // y(empty set), N values
                  unsigned long long* __restrict__ partial,  // [gridDim.x][2]: T mod 2^128 (lo, hi)

@@ -140,7 +140,7 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
 // nonzero, from order 26 (below, a permanent is too small a job for its
 // per-chunk start).  Against the FP64 walk on 0/1 matrices: equal at order
 // 24 (4 entries per row), 1.5x at order 28, 1.2x on average on the config-3
-// leaves of order 30 and 2.6x on the whole order-40 matrix; with larger
+// leaves of order 30 and 2.5x on the whole order-40 matrix; with larger
 // entries it replaces the int32 walk (2.3x faster on order-30 leaves of
 // the curtis54 example).
 bool int_sparse_preferred(const int64_t* a, int n)
