@@ -12,8 +12,14 @@ namespace spm {
 
 namespace ints {
 
+// Orders 36..40 ask the register allocator for 4 waves per SIMD: their
+// natural allocation (131-137 VGPRs) leaves 3, and at 128 the spills fall
+// outside the step loops.
 template <int N>
-__global__ void __launch_bounds__(kDenseBlock)
+constexpr int kIntWaves = (N >= 36 && N <= 40) ? 4 : 1;
+
+template <int N>
+__global__ void __launch_bounds__(kDenseBlock) __attribute__((amdgpu_waves_per_eu(kIntWaves<N>)))
 int_dense_kernel(const int* __restrict__ M,        // [2][N][N]: +2A then -2A, column-major
                  const int* __restrict__ y0,       // y(empty set), N values
                  unsigned long long* __restrict__ partial,  // [gridDim.x][2]: T mod 2^128 (lo, hi)
