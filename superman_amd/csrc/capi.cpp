@@ -99,7 +99,15 @@ double now_seconds()
 // (tasks >= resident waves), large enough that rebuilding x at a chunk start
 // (~n (n-L) FMAs plus the task bookkeeping) is <~0.1% of the chunk's
 // 2^L (2n-1) ops (measured at n=40: 2^10-step chunks cost 3% more time).
-void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks)
+// Where the wave count sets L (small orders), a chunk start -- n columns
+// through dependent scalar loads -- still outweighs a few more idle wave
+// slots, so the dense walks take longer chunks there (`walk`): real +2
+// while that L is at most 5, +1 up to 8; complex +1 up to 5 (measured:
+// real n=22 24.1 -> 15.5 us, n=24 35.3 -> 27.2 us, n=26 78.5 -> 73.8 us;
+// complex n=22 36.8 -> 32.8 us, n=24 70.8 -> 67.6 us; unchanged or worse
+// beyond).
+void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks,
+                   GrayWalk walk)
 {
     const int B = n - 1;
     // L >= 1 whenever there is a column to flip: the kernel takes the term
@@ -112,6 +120,12 @@ void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint6
     while ((int64_t(1) << lw) < std::max<int64_t>(resident_waves, 1)) ++lw;
     const int Lp = B - 6 - lw;
     int l = std::max(lo, std::min(La, Lp));
+    if (Lp < La) {
+        int extra = 0;
+        if (walk == GrayWalk::kReal) extra = Lp <= 5 ? 2 : Lp <= 8 ? 1 : 0;
+        else if (walk == GrayWalk::kComplex) extra = Lp <= 5 ? 1 : 0;
+        l = std::min(La, l + extra);
+    }
     l = std::min(l, B);
     *L = l;
     *chunks = uint64_t(1) << (B - l);
@@ -215,7 +229,7 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     }
     const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
     const int ub = kind == 1 ? dense_f64_unroll_bits(n) : (n >= 3 ? 1 : 0);
-    gray_geometry(n, ub, waves, &p->L, &p->chunks, &p->tasks);
+    gray_geometry(n, ub, waves, &p->L, &p->chunks, &p->tasks, kind == 1 ? GrayWalk::kReal : GrayWalk::kComplex);
     const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
     p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
     // Split very long runs (~> 4 s of VALU work at 64 lanes x 256 CUs x 2.4 GHz)
@@ -716,7 +730,7 @@ extern "C" int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk,
                                  int* unroll_bits)
 {
     if (n < 1 || n > kMaxN || !log2_chunk || !chunks || !tasks) return SPM_EINVAL;
-    gray_geometry(n, dense_f64_unroll_bits(n), resident_waves, log2_chunk, chunks, tasks);
+    gray_geometry(n, dense_f64_unroll_bits(n), resident_waves, log2_chunk, chunks, tasks, GrayWalk::kReal);
     if (unroll_bits) *unroll_bits = dense_f64_unroll_bits(n);
     return SPM_OK;
 }

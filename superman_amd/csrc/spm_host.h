@@ -20,7 +20,11 @@ double now_seconds();
 
 // Gray-walk geometry shared by the dense kernels: 2^L steps per lane chunk,
 // 2^(n-1-L) chunks, grouped 64 per wave task.
-void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks);
+// Which walk a geometry is for: the dense walks take longer chunks than the
+// wave count alone would give at small orders.
+enum class GrayWalk { kOther, kReal, kComplex };
+void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks,
+                   GrayWalk walk = GrayWalk::kOther);
 
 // A-priori absolute error estimate for a compensated FP64 real run:
 // each term carries ~(n-1) product roundings plus the rounding of its n
