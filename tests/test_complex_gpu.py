@@ -90,3 +90,15 @@ def test_orders_with_three_column_parts(n):
     rr = api.dense_f64(b)
     assert abs(rc.value - rr.value) <= rc.err_est + rr.err_est
     assert abs(rc.value.imag) == 0.0
+
+
+@pytest.mark.parametrize("n", [22, 23, 26])
+def test_small_order_chunks_invariance(n):
+    # orders whose chunk length comes from the small-order rule (see the
+    # real walk's test): transpose and permutations agree within the estimates
+    a = W.complex_gauss(n, 400 + n)
+    rng = np.random.default_rng(n)
+    b = a[rng.permutation(n)][:, rng.permutation(n)]
+    r, rt, rp = api.dense_c128(a), api.dense_c128(np.ascontiguousarray(a.T)), api.dense_c128(b)
+    for s in (rt, rp):
+        assert abs(s.value - r.value) <= r.err_est + s.err_est

@@ -97,3 +97,16 @@ def test_n40_properties(golden):
     # transpose: different arithmetic, same permanent
     t = api.dense_f64(np.ascontiguousarray(a.T), "kahan")
     assert rel_err(t.value, comp.value) <= TOL
+
+
+@pytest.mark.parametrize("n", [23, 25, 26, 27])
+def test_small_order_chunks_invariance(n):
+    # orders whose chunk length comes from the small-order rule: transpose
+    # and row/column permutations walk the columns in other orders and must
+    # agree within the stated estimates
+    a = W.dense_normal(n, 300 + n)
+    rng = np.random.default_rng(n)
+    b = a[rng.permutation(n)][:, rng.permutation(n)]
+    r, rt, rp = api.dense_f64(a, "kahan"), api.dense_f64(np.ascontiguousarray(a.T), "kahan"), api.dense_f64(b, "kahan")
+    for s in (rt, rp):
+        assert abs(s.value - r.value) <= r.err_est + s.err_est
