@@ -105,7 +105,9 @@ double now_seconds()
 // while that L is at most 5, +1 up to 8; complex +1 up to 5 (measured:
 // real n=22 24.1 -> 15.5 us, n=24 35.3 -> 27.2 us, n=26 78.5 -> 73.8 us;
 // complex n=22 36.8 -> 32.8 us, n=24 70.8 -> 67.6 us; unchanged or worse
-// beyond).
+// beyond).  The exact integer walks and the generated kernels keep the
+// wave-count chunk: the same rule gained only at order 22 for them and lost
+// 5-45% at orders 20, 26 and 28.
 void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks,
                    GrayWalk walk)
 {
