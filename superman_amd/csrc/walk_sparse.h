@@ -173,10 +173,11 @@ __device__ __forceinline__ unsigned __int128 row_product(const int (&y)[N], bool
 // its row sums and sign into the wave's LDS queue, and when the queue is
 // full every lane forms one queued product.  Entries are kQStride ints (N
 // rows, the sign), an odd stride so that 64 lanes reading one entry each
-// touch distinct banks.  A queue holds 64 entries, except at orders 39..44,
+// touch distinct banks.  A queue holds 64 entries, except at orders 39..41,
 // where 64 would hold the kernel to 3 blocks of 4 waves per CU although
-// its registers allow 4: there it holds what fits in a sixteenth of the
-// CU's 160 KiB LDS less the block's reduction records.
+// its registers allow 4 (at 128 VGPRs the spills fall around the flush's
+// products): there it holds what fits in a sixteenth of the CU's 160 KiB
+// LDS less the block's reduction records.
 template <int N>
 constexpr int kQStride = (N + 1) | 1;
 constexpr int kCuLdsBytes = 160 * 1024;
@@ -184,7 +185,7 @@ constexpr int kQBlocksPerCu = 4;
 constexpr int kQBlockReserve = 512;  // the block's reduction records, rounded to the allocation unit
 template <int N>
 constexpr uint32_t kQCap =
-    (N >= 39 && N <= 44)
+    (N >= 39 && N <= 41)
         ? (uint32_t)((kCuLdsBytes / kQBlocksPerCu - kQBlockReserve) / ((kDenseBlock / kWave) * kQStride<N> * 4))
         : 64u;
 template <int N>
