@@ -109,13 +109,18 @@ namespace {
 
 constexpr int kGenU = 3;  // log2 of the steps per generated block
 
-// Orders whose kernels are asked for 5 waves per SIMD: from order 29 the
-// natural register allocation (100-127 VGPRs) leaves 4; at 96 VGPRs the
-// compiler spills only around the term product and the final reduction
-// (config-3 order-40 pattern: 155 -> 141 ms).  Orders up to 28 reach 5
-// without it; past 40 more rows would spill into the steps.
-constexpr int kGenWaves5Min = 29;
-constexpr int kGenWaves5Max = 40;
+// Waves per SIMD asked of the register allocator (0: the compiler's
+// choice).  Orders 29..40: the natural allocation (100-127 VGPRs) leaves 4;
+// at 96 VGPRs the compiler spills only around the term product and the
+// final reduction (config-3 order-40 pattern: 155 -> 141 ms).  Orders
+// 41..48: 129-130 VGPRs leave 3; 128 spill at most 8 bytes (order-42
+// pattern: 706 -> 590 ms).  Orders up to 28 reach 5 without it.
+int gen_waves_per_simd(int n)
+{
+    if (n >= 29 && n <= 40) return 5;
+    if (n >= 41 && n <= 48) return 4;
+    return 0;
+}
 
 std::string lit(int64_t v)
 {
@@ -177,7 +182,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
     std::ostringstream o;
     o << "// generated: SpaRyser kernel for one " << n << "x" << n << " integer pattern\n";
     o << "extern \"C\" __global__ void __launch_bounds__(256)";
-    if (n >= kGenWaves5Min && n <= kGenWaves5Max) o << " __attribute__((amdgpu_waves_per_eu(5)))";
+    if (const int w = gen_waves_per_simd(n)) o << " __attribute__((amdgpu_waves_per_eu(" << w << ")))";
     o << "\n";
     o << name << "(unsigned long long* __restrict__ partial, unsigned long long task_begin,\n"
       << "    unsigned long long task_end, unsigned long long num_chunks, int L, int accumulate)\n{\n";
