@@ -40,39 +40,124 @@ METRIC = "dense n=40 perm time & % FP64 VALU peak; batched perms/s + p99 latency
 PEAK_FP64_TFLOPS = 78.6
 
 
-def host_cores() -> int:
+def host_threads() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS where the host sets it
+    (the CPU share that goes with one GPU), else every CPU this process may
+    run on."""
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        aff = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return max(1, min(int(env), aff))
+    return max(1, aff)
+
+
+def host_info() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "threads_used": host_threads(), "cpu_model": model,
+            "thread_rule": "OMP_NUM_THREADS when set (the CPU share of one GPU on a shared host), else all CPUs "
+                           "in this process's affinity mask"}
+
+
+def _median_time(fn, runs: int = 3):
+    ts, v = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        v = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), v
+
+
+def _sized_sample(fn, n: int, target_s: float) -> tuple[int, float]:
+    """Runs fn(count) on a short prefix, then on a prefix sized to about
+    target_s seconds (at most the whole 2^(n-1) range); (count, seconds)."""
+    count = 1 << 20
+    t0 = time.perf_counter()
+    fn(count)
+    dt = time.perf_counter() - t0
+    count = int(min((1 << (n - 1)) - 1, max(count, count * target_s / max(dt, 1e-6))))
+    t0 = time.perf_counter()
+    fn(count)
+    return count, time.perf_counter() - t0
 
 
 def cpu_baseline(a: np.ndarray, target_s: float = 12.0) -> dict:
+    """The reference's CPU algorithms on this host: the headline n=40 dense
+    permanent as a bounded sample of the restated walk (a whole one takes
+    hours), the n=24 dense (whole), complex Haar n=32 (sample) and sparse
+    0/1 n=40 (whole, decomposed by the reference itself) configurations."""
     from oracle import oracle as O  # CPU baseline timing only
+    from oracle import refbin
+    from superman_amd import workloads as W
     n = a.shape[0]
-    cores = host_cores()
-    count = 1 << 22
-    t0 = time.perf_counter()
-    O.dense_ref_sample(a, cores, count)
-    dt = time.perf_counter() - t0
-    count = int(min(1 << (n - 1), max(count, count * target_s / max(dt, 1e-6))))
-    t0 = time.perf_counter()
-    O.dense_ref_sample(a, cores, count)
-    dt = time.perf_counter() - t0
+    threads = host_threads()
+    count, dt = _sized_sample(lambda c: O.dense_ref_sample(a, threads, c), n, target_s)
     rate = count / dt  # subsets per second
-    out = {"value": rate / float(1 << (n - 1)), "unit": "perm/s", "cores": cores, "kind": "port",
+    out = {"value": rate / float(1 << (n - 1)), "unit": "perm/s", "cores": threads, "kind": "port",
            "sample": f"reference dpNaivePerman walk (oracle/ryser_oracle.c orc_dense_ref_sample) over "
-                     f"Gray indices [1, 1+{count}) of the same n={n} matrix on {cores} threads, {dt:.1f} s; "
-                     f"{rate:.3e} subsets/s scaled to 2^{n - 1} subsets per permanent"}
-    out["reference_check"] = reference_check(cores)
+                     f"Gray indices [1, 1+{count}) of the same n={n} matrix on {threads} threads, {dt:.1f} s; "
+                     f"{rate:.3e} subsets/s ({rate / threads:.3e} per thread) scaled to 2^{n - 1} subsets "
+                     f"per permanent",
+           "host": host_info()}
+    have_ref = refbin.available()
+    # config 1: dense uniform n=24, one whole permanent, median of 3
+    c1 = W.dense_uniform(24, 24)
+    port_s, port_v = _median_time(lambda: O.dense_ref(c1, threads))
+    d1 = {"port_s_median": port_s, "port_value": port_v, "threads": threads, "runs": 3}
+    if have_ref:
+        rts = [refbin.run(c1, threads=threads) for _ in range(3)]
+        d1.update({"reference_s_median": float(np.median([r[1] for r in rts])), "reference_value": rts[-1][0],
+                   "reference_timer": "the binary's own 'Computed in' seconds (main.cpp:122-131)"})
+    out["c1_dense_uniform_n24"] = d1
+    # config 4: Haar complex n=32, bounded sample of the restated complex walk
+    z = W.haar_block(32, 32)
+    zc, zdt = _sized_sample(lambda c: O.complex_ref_sample(z, threads, c), 32, 5.0)
+    out["c4_complex_haar_n32"] = {
+        "perm_per_s": zc / zdt / float(1 << 31), "seconds_per_perm": float(1 << 31) * zdt / zc,
+        "kind": "port", "threads": threads,
+        "sample": f"cpuComputeComplex walk (orc_complex_ref_sample) over Gray indices [1, 1+{zc}) on "
+                  f"{threads} threads, {zdt:.1f} s, scaled to 2^31 subsets"}
+    # config 3: sparse 0/1 n=40 through the reference's own pipeline (DM
+    # pruning, components, compressions, CPU sparse walk), median of 3
+    if have_ref:
+        b = W.sparse01(40, 40)
+        rts = [refbin.run(b, threads=threads) for _ in range(3)]
+        out["c3_sparse01_n40"] = {"reference_s_median": float(np.median([r[1] for r in rts])),
+                                  "reference_value": rts[-1][0], "kind": "reference", "threads": threads}
+    rc = reference_check(threads)
+    out["reference_check"] = rc
+    if rc is not None:
+        # the reference binary's complex walk is slower than the port (its
+        # std::complex products go through the C runtime's NaN-checking
+        # multiply), so config 4's baseline is the binary's whole-permanent
+        # rate at n=24 scaled by subsets x order to n=32
+        c = rc["complex"]
+        t32 = c["reference_s"] * float(1 << (31 - (c["n"] - 1))) * 32.0 / c["n"]
+        out["c4_complex_haar_n32"].update({"reference_seconds_per_perm_extrapolated": t32,
+                                           "reference_perm_per_s_extrapolated": 1.0 / t32})
     return out
 
 
-def reference_check(cores: int, n: int = 30) -> dict | None:
-    """The reference's own CPU binary (oracle/_ref, built from its sources) and
-    the port on one whole n=30 permanent of the same generator, same threads:
-    the port's speed stands in for the reference's on the n=40 sample."""
+def reference_check(cores: int, n: int = 30, nc: int = 24) -> dict | None:
+    """The reference's own CPU binary (oracle/_ref, built from its sources)
+    and the port on one whole n=30 dense permanent and one whole n=24 complex
+    permanent of the same generators, same threads: the port's speed stands
+    in for the reference's on the n=40 and complex n=32 samples."""
     from oracle import oracle as O
     from oracle import refbin
     if not refbin.available():
@@ -84,9 +169,18 @@ def reference_check(cores: int, n: int = 30) -> dict | None:
     t_port = time.perf_counter() - t0
     v_ref, t_ref, _ = refbin.run(a, threads=cores)
     sub = float(1 << (n - 1))
+    z = W.complex_gauss(nc, nc)
+    t0 = time.perf_counter()
+    z_port = O.complex_ref(z, cores)
+    tz_port = time.perf_counter() - t0
+    z_ref, tz_ref, _ = refbin.run(z, threads=cores)
+    subc = float(1 << (nc - 1))
     return {"n": n, "threads": cores, "reference_s": t_ref, "port_s": t_port,
             "reference_subsets_per_s": sub / t_ref, "port_subsets_per_s": sub / t_port,
-            "values_rel_diff": abs(v_port - v_ref) / abs(v_ref)}
+            "values_rel_diff": abs(v_port - v_ref) / abs(v_ref),
+            "complex": {"n": nc, "reference_s": tz_ref, "port_s": tz_port,
+                        "reference_subsets_per_s": subc / tz_ref, "port_subsets_per_s": subc / tz_port,
+                        "values_rel_diff": abs(z_port - z_ref) / abs(z_ref)}}
 
 
 def measured_traffic(kernel: str):
@@ -139,6 +233,11 @@ def serving(count: int, seed: int = 2026, load: float = 0.8) -> dict:
         rc, res, lat, st = eng.replay(reqs, unit_arrivals / rate)
         if rc != 0:
             raise RuntimeError(f"batched Poisson run failed: status {rc} ({eng.last_error()})")
+        health = eng.counters()
+    if health["late_publish"] or health["retried"]:
+        # a result that reached the host late or had to be run again is a
+        # defect of the engine, not noise: the run fails
+        raise RuntimeError(f"batch engine health: {health} ({eng.last_error()})")
     batched = [to_permanent(r, q[0]) for r, q in zip(res, reqs)]
 
     # plain path: one C-ABI call per matrix, back to back (closed loop)

@@ -81,10 +81,10 @@ inline int precision_of(const Settings& s)
 
 // DecomposePerman<..., dpSingleGPU / spSingleGPU> (DecomposePerman.h:76-117
 // around dpSingleGPU.cuh:44-219 / spSingleGPU.cuh:44-234): compressions
-// above order 30, then every leaf on the GPU -- exactly for integer leaves
-// (spm_int_exact), else in FP64 with the requested accumulation.  All dense
-// Algorithm values map to the one dense kernel, all sparse ones to the
-// integer walks.
+// above order 30, then every leaf on the GPU by the automatic path choice
+// (spm_permanent_auto): exact for integer and row-scaled integer leaves,
+// else FP64 with the requested accumulation in a generated kernel or the
+// dense walk, whichever is estimated faster.
 template <class C, class S>
 class DecomposedGPU : public Permanent<C, S>
 {
@@ -143,6 +143,12 @@ public:
                 superman_amd::check(st, "spm_sparse_codegen");
                 return last.re;
             }
+        } else if (n >= 5) {  // real values: the generated FP64 kernel (exact hex-float literals)
+            const int st = spm_sparse_codegen_real(b.data(), n, opt, superman_amd::precision_of(this->m_Settings),
+                                                   this->m_Settings.deviceID, nullptr, &last);
+            delete leaf;
+            superman_amd::check(st, "spm_sparse_codegen_real");
+            return last.re;
         }
         DecomposedGPU<C, S> generic(leaf, this->m_Settings);
         const double v = generic.computePermanent();

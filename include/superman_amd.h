@@ -130,6 +130,20 @@ int spm_codegen_source(const int64_t* a, int n, int options, char* buf, size_t b
 /* hiprtc-compiles the generated kernel without loading it (no device
  * needed): compile time and code-object size. */
 int spm_codegen_compile_only(const int64_t* a, int n, int options, double* seconds, size_t* code_bytes);
+/* Generated SpaRyser kernels for real (FP64) matrices.  Same geometry and
+ * generator options as spm_sparse_codegen; `precision` is SPM_PREC_PLAIN or
+ * SPM_PREC_COMPENSATED.  Literals are the entries' exact binary64 values
+ * (hex floats; the reference prints 6-decimal std::to_string literals,
+ * UTOrderedKernelCodeGen.cuh:292-295), each flip updates only the rows of
+ * the flipped column, and the product reuses cached products of the rows a
+ * flip cannot have changed (the reference's register-area split of rows
+ * kept in registers and a cached globalProduct, UTOrderedKernelCodeGen.cuh:
+ * 68-132, 365-454).  Replaces KernelGenSingleGPU (KernelGenSingleGPU.cuh:
+ * 47-197) for non-integer matrices; err_est / abs_sum as for spm_dense_f64. */
+int spm_sparse_codegen_real(const double* a, int n, int options, int precision, int device, void* stream,
+                            spm_result* out);
+int spm_codegen_source_real(const double* a, int n, int options, int precision, char* buf, size_t buflen,
+                            size_t* needed);
 /* IO::UTOrder (IO/IO.h:926-1002) on the pattern of `a`: the permuted matrix
  * is b[row_perm[i]][col_perm[j]] = a[i][j] (IO::applyPermutations,
  * IO/IO.h:1095-1112).  Host only. */
@@ -157,6 +171,8 @@ int spm_plan_int(const int64_t* a, int n, int device, spm_plan** plan);
 int spm_plan_int_walk(const int64_t* a, int n, int walk, int device, spm_plan** plan);
 /* Compiles (or takes from the cache) the generated kernel for `a`. */
 int spm_plan_codegen(const int64_t* a, int n, int options, int device, spm_plan** plan);
+/* The same for a real matrix (spm_sparse_codegen_real). */
+int spm_plan_codegen_real(const double* a, int n, int options, int precision, int device, spm_plan** plan);
 int spm_plan_codegen_info(const spm_plan* plan, double* compile_seconds, int* cached, uint64_t* hash);
 int spm_plan_run(spm_plan* plan, void* stream, spm_result* out);
 int spm_plan_destroy(spm_plan* plan);
@@ -172,6 +188,36 @@ int spm_gray_geometry(int n, int64_t resident_waves, int* log2_chunk, uint64_t* 
                       int* unroll_bits);
 /* Name of the permanent kernel a plan launches (for profiler lookups). */
 const char* spm_plan_kernel_name(const spm_plan* plan);
+/* Dense plans only: walk chunks of 2^log2_chunk Gray steps instead of the
+ * automatic length (U + 1 <= log2_chunk <= n - 1, U = the kernel's unroll
+ * bits).  A different chunk length starts x from an exact rebuild at other
+ * places, so the rounding differs while the permanent does not (used by
+ * accuracy tests and tuning sweeps).  No reference counterpart: the
+ * reference's chunk follows from its thread count (dpSingleGPU.cuh:148-199). */
+int spm_plan_set_log2_chunk(spm_plan* plan, int log2_chunk);
+
+/* ---- automatic path choice ------------------------------------------------
+ * One real matrix by the path whose estimated time (rates measured on
+ * MI355X per Gray subset, plus the hiprtc compile of a generated kernel)
+ * is smallest -- the MI355X counterpart of the reference's automatic
+ * algorithm choice (CommonFiles/AlgorithmSelector.h:52-67), which weighs
+ * nnz per row and component size on its own GPUs:
+ *   integer entries        exact walk or exact generated kernel;
+ *   rows s_i * integers    exact integer permanent times prod s_i, rounded
+ *                          once (err_est = 2^-52 |value|);
+ *   other real entries     generated FP64 kernel or the dense FP64 walk.
+ * `path_taken` (may be NULL) receives the SPM_PATH_* value that ran;
+ * spm_choose_path gives the choice without running it (host only) and
+ * spm_path_estimate one path's estimated seconds (compile included). */
+#define SPM_PATH_INT        1
+#define SPM_PATH_GEN_INT    2
+#define SPM_PATH_SCALED_INT 3
+#define SPM_PATH_GEN_REAL   4
+#define SPM_PATH_DENSE      5
+int spm_permanent_auto(const double* a, int n, int precision, int device, void* stream, spm_result* out,
+                       int* path_taken);
+int spm_choose_path(const double* a, int n, int* path);
+int spm_path_estimate(const double* a, int n, int path, double* seconds);
 
 /* ---- structural preprocessing (sparse inputs) ---------------------------------
  * The reference's pipeline in front of its kernels, with the same
@@ -316,6 +362,11 @@ int spm_serve_pending(spm_batch* b, int* queued, int* in_flight);
  * out[i] and latency_s[i] (arrival to completion) and the stream summary. */
 int spm_serve_replay(spm_batch* b, int count, const spm_request* reqs, const double* arrival_s, spm_result* out,
                      double* latency_s, spm_serve_stats* stats);
+/* Engine-lifetime totals of late_publish and retried (spm_serve_stats); both
+ * stay 0 unless a result reached host memory after its batch's completion
+ * or failed a check.  No reference counterpart (a health check for callers
+ * of the batch engine). */
+int spm_batch_counters(spm_batch* b, uint64_t* late_publish, uint64_t* retried);
 /* The next `launches` batch launches fail on the host without touching the
  * device; their requests complete with SPM_EFAULTINJ. */
 int spm_batch_inject_fault(spm_batch* b, int launches);

@@ -45,6 +45,8 @@ def lib() -> ctypes.CDLL:
         L.orc_sparse_ref.restype = ctypes.c_double
         L.orc_complex_ref.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_complex_ref.restype = None
+        L.orc_complex_ref_sample.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, dp]
+        L.orc_complex_ref_sample.restype = None
         L.orc_dense_dd.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.orc_dense_dd.restype = None
         L.orc_complex_dd.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp]
@@ -108,6 +110,15 @@ def complex_ref(a, threads: int = 1) -> complex:
     a = _cplx(a)
     out = np.zeros(2)
     lib().orc_complex_ref(_dptr(a.view(np.float64)), a.shape[0], threads, _dptr(out))
+    return complex(out[0], out[1])
+
+
+def complex_ref_sample(a, threads: int, count: int) -> complex:
+    """The restated reference complex walk over Gray indices [1, 1+count)
+    only (bounded CPU-baseline sample); returns the unscaled partial sum."""
+    a = _cplx(a)
+    out = np.zeros(2)
+    lib().orc_complex_ref_sample(_dptr(a.view(np.float64)), a.shape[0], threads, count, _dptr(out))
     return complex(out[0], out[1])
 
 

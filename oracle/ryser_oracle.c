@@ -18,6 +18,9 @@
  *                    Permanent::computePermanent (CommonFiles/Permanent.h:64-91)
  *                    feeds it in CPU mode.
  *   orc_complex_ref  restatement of Algorithms/Complex/cpuComputeComplex.h:34-142.
+ *   orc_dense_ref_sample / orc_complex_ref_sample
+ *                    the same two walks over a prefix [1, 1+count) of the
+ *                    Gray indices (bounded CPU-baseline samples).
  *   orc_dense_dd     double-double Ryser (x, products and sums carried as
  *                    unevaluated double pairs); accuracy oracle for FP64.
  *   orc_complex_dd   the same for complex matrices.
@@ -332,6 +335,74 @@ void orc_complex_ref(const double *a, int nov, int threads, double *out)
     double f = (double)(4 * (nov % 2) - 2);
     out[0] = f * sumr;
     out[1] = f * sumi;
+}
+
+/* Bounded sample of the complex walk, for the CPU baseline: Gray indices
+ * [1, 1 + count) split over `threads` as cpuComputeComplex.h:84-87 splits
+ * [1, 2^(n-1)), the same arithmetic as orc_complex_ref.  out = the unscaled
+ * partial sum (re, im), so the work cannot be optimised away. */
+void orc_complex_ref_sample(const double *a, int nov, int threads, long long count, double *out)
+{
+    out[0] = out[1] = NAN;
+    if (nov < 2 || nov > ORC_MAX_N || threads < 1 || count < 1) return;
+    double xr[ORC_MAX_N], xi[ORC_MAX_N];
+    for (int i = 0; i < nov; ++i) {
+        double sr = 0.0, si = 0.0;
+        for (int j = 0; j < nov; ++j) { sr += a[2 * (i * nov + j)]; si += a[2 * (i * nov + j) + 1]; }
+        xr[i] = a[2 * (i * nov + nov - 1)] - sr / 2.0;
+        xi[i] = a[2 * (i * nov + nov - 1) + 1] - si / 2.0;
+    }
+    double *mt = (double *)malloc(sizeof(double) * 2 * nov * nov);
+    for (int i = 0; i < nov; ++i)
+        for (int j = 0; j < nov; ++j) {
+            mt[2 * (j * nov + i)] = a[2 * (i * nov + j)];
+            mt[2 * (j * nov + i) + 1] = a[2 * (i * nov + j) + 1];
+        }
+    const long long start = 1;
+    long long end = 1LL << (nov - 1);
+    if (start + count < end) end = start + count;
+    double sumr = 0.0, sumi = 0.0;
+#pragma omp parallel num_threads(threads)
+    {
+        double rr = 0.0, ri = 0.0;
+        double mxr[ORC_MAX_N], mxi[ORC_MAX_N];
+        memcpy(mxr, xr, sizeof(double) * nov);
+        memcpy(mxi, xi, sizeof(double) * nov);
+        long long chunk = (end - start + threads - 1) / threads;
+        int tid = omp_get_thread_num();
+        long long myStart = start + tid * chunk;
+        long long myEnd = start + (tid + 1) * chunk;
+        if (myEnd > end) myEnd = end;
+        long long gray = (myStart - 1) ^ ((myStart - 1) >> 1);
+        for (int j = 0; j < nov - 1; ++j)
+            if ((gray >> j) & 1LL)
+                for (int i = 0; i < nov; ++i) { mxr[i] += mt[2 * (j * nov + i)]; mxi[i] += mt[2 * (j * nov + i) + 1]; }
+        double sgn = (myStart & 1LL) ? -1.0 : 1.0;
+        for (long long i = myStart; i < myEnd; ++i) {
+            int col = __builtin_ctzll(i);
+            gray ^= (1LL << col);
+            double added = ((1LL << col) & gray) ? 1.0 : -1.0;
+            double qr = 1.0, qi = 0.0;
+            for (int r = 0; r < nov; ++r) {
+                mxr[r] += added * mt[2 * (col * nov + r)];
+                mxi[r] += added * mt[2 * (col * nov + r) + 1];
+                double tr = qr * mxr[r] - qi * mxi[r];
+                double ti = qr * mxi[r] + qi * mxr[r];
+                qr = tr; qi = ti;
+            }
+            rr += sgn * qr;
+            ri += sgn * qi;
+            sgn = -sgn;
+        }
+#pragma omp critical
+        {
+            sumr += rr;
+            sumi += ri;
+        }
+    }
+    free(mt);
+    out[0] = sumr;
+    out[1] = sumi;
 }
 
 /* ------------------------------------------------------------------------ */

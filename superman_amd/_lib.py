@@ -31,6 +31,12 @@ SPM_PREC_PLAIN = 0
 SPM_PREC_COMPENSATED = 1
 SPM_PREC_F32_PRODUCT = 2
 
+SPM_PATH_INT = 1
+SPM_PATH_GEN_INT = 2
+SPM_PATH_SCALED_INT = 3
+SPM_PATH_GEN_REAL = 4
+SPM_PATH_DENSE = 5
+
 SPM_INT_WALK_AUTO = 0
 SPM_INT_WALK_INT32 = 1
 SPM_INT_WALK_FP64 = 2
@@ -129,6 +135,15 @@ PROTOTYPES = {
                                           ctypes.POINTER(ctypes.c_size_t)]),
     "spm_codegen_compile_only": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                                 ctypes.POINTER(ctypes.c_size_t)]),
+    "spm_sparse_codegen_real": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
+    "spm_codegen_source_real": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                               ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "spm_plan_codegen_real": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(_VP)]),
+    "spm_permanent_auto": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R,
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "spm_choose_path": (ctypes.c_int, [_D, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "spm_path_estimate": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "spm_ut_order": (ctypes.c_int, [_D, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "spm_plan_codegen": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
     "spm_plan_codegen_info": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
@@ -140,6 +155,7 @@ PROTOTYPES = {
     "spm_plan_info": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)]),
     "spm_plan_kernel_name": (ctypes.c_char_p, [_VP]),
+    "spm_plan_set_log2_chunk": (ctypes.c_int, [_VP, ctypes.c_int]),
     "spm_gray_geometry": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_int)]),
@@ -165,6 +181,7 @@ PROTOTYPES = {
     "spm_serve_pending": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "spm_serve_replay": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(SpmRequest), _D, _R, _D,
                                         ctypes.POINTER(SpmServeStats)]),
+    "spm_batch_counters": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "spm_batch_inject_fault": (ctypes.c_int, [_VP, ctypes.c_int]),
     "spm_batch_inject_bad_results": (ctypes.c_int, [_VP, ctypes.c_int]),
     "spm_batch_last_error": (ctypes.c_char_p, [_VP]),

@@ -124,14 +124,41 @@ def permanent(a, precision="auto", device: int = 0) -> Permanent:
     if precision == "exact":
         return int_exact(a, device)
     if precision == "auto":
-        if is_integral(a):
-            try:
-                return int_exact(a, device)
-            except _lib.SpmError as e:
-                if e.status != _lib.SPM_ERANGE:
-                    raise
-        precision = "kahan"
+        return permanent_auto(a, "kahan", device)[0]
     return dense_f64(a, precision, device)
+
+
+PATHS = {_lib.SPM_PATH_INT: "int", _lib.SPM_PATH_GEN_INT: "gen_int", _lib.SPM_PATH_SCALED_INT: "scaled_int",
+         _lib.SPM_PATH_GEN_REAL: "gen_real", _lib.SPM_PATH_DENSE: "dense"}
+
+
+def permanent_auto(a, precision="kahan", device: int = 0, stream: int | None = None) -> tuple[Permanent, str]:
+    """The automatically chosen path (spm_permanent_auto): (result, path name)."""
+    a = _square(a, np.float64)
+    r = SpmResult()
+    path = ctypes.c_int()
+    check(lib().spm_permanent_auto(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0], _prec(precision),
+                                   device, stream, ctypes.byref(r), ctypes.byref(path)), "spm_permanent_auto")
+    return _to_perm(r, False), PATHS[path.value]
+
+
+def choose_path(a) -> str:
+    """The path spm_permanent_auto would take (host only, no device)."""
+    a = _square(a, np.float64)
+    path = ctypes.c_int()
+    check(lib().spm_choose_path(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0], ctypes.byref(path)),
+          "spm_choose_path")
+    return PATHS[path.value]
+
+
+def path_estimate(a, path: str) -> float:
+    """Estimated seconds of one path for a (compile time included)."""
+    a = _square(a, np.float64)
+    inv = {v: k for k, v in PATHS.items()}
+    sec = ctypes.c_double()
+    check(lib().spm_path_estimate(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0], inv[path],
+                                  ctypes.byref(sec)), "spm_path_estimate")
+    return sec.value
 
 
 class DensePlan:
@@ -164,6 +191,11 @@ class DensePlan:
         r = SpmResult()
         check(lib().spm_plan_run(self._h, stream, ctypes.byref(r)), "spm_plan_run")
         return _to_perm(r, self.complex)
+
+    def set_log2_chunk(self, log2_chunk: int) -> None:
+        """Dense plans: walk chunks of 2^log2_chunk Gray steps (same
+        permanent, other rounding)."""
+        check(lib().spm_plan_set_log2_chunk(self._h, int(log2_chunk)), "spm_plan_set_log2_chunk")
 
     def info(self) -> dict:
         g, b, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -245,6 +277,31 @@ def codegen_compile_only(a, register_efficient: bool = True) -> dict:
     return {"seconds": sec.value, "code_bytes": nbytes.value}
 
 
+def sparse_codegen_real(a, precision="kahan", register_efficient: bool = True, device: int = 0,
+                        stream: int | None = None) -> Permanent:
+    """FP64 permanent through a kernel generated for a's pattern and exact
+    values (hiprtc): flips update only the flipped column's rows."""
+    a = _square(a, np.float64)
+    r = SpmResult()
+    opt = _lib.SPM_CG_REGISTER_EFFICIENT if register_efficient else 0
+    check(lib().spm_sparse_codegen_real(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0], opt,
+                                        _prec(precision), device, stream, ctypes.byref(r)), "spm_sparse_codegen_real")
+    return _to_perm(r, False)
+
+
+def codegen_source_real(a, precision="kahan", register_efficient: bool = True) -> str:
+    a = _square(a, np.float64)
+    opt = _lib.SPM_CG_REGISTER_EFFICIENT if register_efficient else 0
+    need = ctypes.c_size_t()
+    p = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    check(lib().spm_codegen_source_real(p, a.shape[0], opt, _prec(precision), None, 0, ctypes.byref(need)),
+          "spm_codegen_source_real")
+    buf = ctypes.create_string_buffer(need.value)
+    check(lib().spm_codegen_source_real(p, a.shape[0], opt, _prec(precision), buf, need.value, None),
+          "spm_codegen_source_real")
+    return buf.value.decode()
+
+
 def ut_order(a) -> tuple[np.ndarray, np.ndarray]:
     """IO::UTOrder on a's pattern: b[row_perm[i], col_perm[j]] = a[i, j]."""
     a = _square(np.asarray(a, dtype=np.float64), np.float64)
@@ -280,6 +337,21 @@ class CodegenPlan(DensePlan):
         sec, cached, h = ctypes.c_double(), ctypes.c_int(), ctypes.c_uint64()
         check(lib().spm_plan_codegen_info(self._h, ctypes.byref(sec), ctypes.byref(cached), ctypes.byref(h)))
         return {"compile_seconds": sec.value, "cached": bool(cached.value), "hash": h.value}
+
+
+class CodegenRealPlan(DensePlan):
+    """A generated SpaRyser kernel for one real matrix (compiled once)."""
+
+    def __init__(self, a, precision="kahan", register_efficient: bool = True, device: int = 0):
+        a = _square(a, np.float64)
+        self.complex = False
+        self._h = ctypes.c_void_p()
+        opt = _lib.SPM_CG_REGISTER_EFFICIENT if register_efficient else 0
+        check(lib().spm_plan_codegen_real(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0], opt,
+                                          _prec(precision), device, ctypes.byref(self._h)), "spm_plan_codegen_real")
+        self.n = a.shape[0]
+
+    codegen_info = CodegenPlan.codegen_info
 
 
 @dataclass

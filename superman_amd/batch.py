@@ -212,6 +212,12 @@ class BatchEngine:
     def last_error(self) -> str:
         return lib().spm_batch_last_error(self._h).decode()
 
+    def counters(self) -> dict:
+        """Engine-lifetime late_publish / retried totals (0 on a healthy run)."""
+        late, retried = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().spm_batch_counters(self._h, ctypes.byref(late), ctypes.byref(retried)))
+        return {"late_publish": late.value, "retried": retried.value}
+
     def inject_fault(self, launches: int = 1) -> None:
         check(lib().spm_batch_inject_fault(self._h, launches))
 

@@ -61,10 +61,31 @@ def _compile(src: str, flags: list[str], verbose: bool) -> tuple[str, float, str
     return obj, time.time() - t, out.stderr
 
 
+def _headers_mtime() -> float:
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    inc = os.path.join(ROOT, "include")
+    for d, _, fs in os.walk(inc):
+        hs += [os.path.join(d, f) for f in fs if f.endswith(".h")]
+    return max((os.path.getmtime(h) for h in hs), default=0.0)
+
+
+def _fresh(src: str, defines) -> bool:
+    """An object newer than its source and every header (incremental
+    builds of the default configuration only)."""
+    if defines:
+        return False
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if not os.path.exists(obj):
+        return False
+    t = os.path.getmtime(obj)
+    return t > os.path.getmtime(os.path.join(CSRC, src)) and t > _headers_mtime()
+
+
 def build(jobs: int = 8, verbose: bool = False, defines=(), out: str | None = None,
-          build_dir: str | None = None) -> str:
+          build_dir: str | None = None, incremental: bool = False) -> str:
     """Compile and link.  `defines` (NAME=VALUE) and `out`/`build_dir` let a
-    tuning sweep build variants of the library away from the package."""
+    tuning sweep build variants of the library away from the package;
+    `incremental` recompiles only sources older than their object."""
     global BUILD
     lib_path = out or LIB
     if build_dir:
@@ -74,7 +95,12 @@ def build(jobs: int = 8, verbose: bool = False, defines=(), out: str | None = No
     t0 = time.time()
     objs = []
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        futs = [ex.submit(_compile, s, f + extra, verbose) for s, f in SOURCES]
+        futs = []
+        for s, f in SOURCES:
+            if incremental and _fresh(s, defines):
+                objs.append(os.path.join(BUILD, os.path.splitext(s)[0] + ".o"))
+            else:
+                futs.append(ex.submit(_compile, s, f + extra, verbose))
         for fu in futs:
             obj, dt, _ = fu.result()
             objs.append(obj)
@@ -120,8 +146,9 @@ def main(argv=None):
     ap.add_argument("-D", dest="defines", action="append", default=[], help="NAME=VALUE for a tuning variant")
     ap.add_argument("--out", help="library path (default: in the package)")
     ap.add_argument("--build-dir", help="object directory (default: superman_amd/_build)")
+    ap.add_argument("--incremental", action="store_true", help="recompile only sources newer than their object")
     a = ap.parse_args(argv)
-    build(a.jobs, a.verbose, a.defines, a.out, a.build_dir)
+    build(a.jobs, a.verbose, a.defines, a.out, a.build_dir, a.incremental)
 
 
 if __name__ == "__main__":

@@ -29,8 +29,21 @@
 //             is below 2^(bit width of n), e.g. 0/1 matrices)
 // and in every record word 7 = batch_tag(seq, item or request index); in a
 // result record word 6 = the number of item records the reduction found
-// without this batch's tag (0 unless the records were not yet visible) and
-// word 5 = record_check() of the other seven words.
+// without this batch's tag and word 5 = record_check() of the other seven
+// words.
+//
+// Memory ordering.  Nothing here depends on a cache being written back or
+// invalidated at the right moment:
+//   - the batch reaches the device arena through batch_stage_kernel, the
+//     first kernel of the batch's launch sequence (a kernel, not a DMA
+//     copy): kernel-to-kernel visibility on one stream is all the walk
+//     kernel relies on;
+//   - item records are written and read with device-scope atomic stores and
+//     loads, and the per-request count that hands a request to its reducing
+//     wave is an acquire-release device-scope atomic;
+//   - a result record is stored to host memory with system-scope stores,
+//     then its flag with a system-scope release store, so a host that reads
+//     the flag with acquire semantics sees the whole record.
 #include "spm_batch.h"
 #include "walk_c128.h"
 #include "walk_f64.h"
@@ -58,6 +71,19 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__bui
 __device__ __forceinline__ uint64_t uniform(uint64_t v)
 {
     return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
+}
+
+// Item-record words: device-scope atomic (coherent) stores and loads.
+__device__ __forceinline__ void store_agent(double* p, double v)
+{
+    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double load_agent(const double* p)
+{
+    return __longlong_as_double(
+        (long long)__hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 template <int N>
@@ -167,7 +193,10 @@ __device__ __forceinline__ void reduce_request(int kind, const double* __restric
     uint32_t stale = 0;
     const bool is_int = kind == kBatchInt || kind == kBatchIntSmall;
     for (uint32_t j = lane; j < items; j += kWave) {
-        const double* rec = recs + (size_t)j * kBatchRec;
+        const double* p = recs + (size_t)j * kBatchRec;
+        double rec[kBatchRec];
+#pragma unroll
+        for (int c = 0; c < kBatchRec; ++c) rec[c] = load_agent(p + c);
         stale += __double_as_longlong(rec[kBatchTag]) != (long long)batch_tag(seq, item_base + j) ? 1u : 0u;
         if (is_int) {
             t += ((unsigned __int128)(unsigned long long)__double_as_longlong(rec[1]) << 64) |
@@ -192,7 +221,9 @@ __device__ __forceinline__ void reduce_request(int kind, const double* __restric
     for (int c = 0; c < kBatchRec; ++c) w[c] = (uint64_t)__double_as_longlong(o[c]);
     o[kBatchCheck] = __longlong_as_double((long long)record_check(w));
 #pragma unroll
-    for (int c = 0; c < kBatchRec; ++c) out[c] = o[c];
+    for (int c = 0; c < kBatchRec; ++c)
+        __hip_atomic_store((unsigned long long*)(out + c), (unsigned long long)__double_as_longlong(o[c]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace batch
@@ -246,33 +277,40 @@ batch_walk_kernel(const uint8_t* __restrict__ arena, double* __restrict__ item_r
         o[kBatchTag] = __longlong_as_double((long long)batch_tag(seq, i));
         double* rec = item_rec + (size_t)i * kBatchRec;
 #pragma unroll
-        for (int c = 0; c < kBatchRec; ++c) rec[c] = o[c];
+        for (int c = 0; c < kBatchRec; ++c) store_agent(rec + c, o[c]);
         // The wave that completes a request's last item reduces its records
         // (release/acquire through the per-request count).
-        __threadfence();
         const uint32_t r = uniform(item_req[i]);
         const uint32_t items = uniform(d->items);
-        const uint32_t prev = uniform(atomicAdd(done + r, lane == 0 ? 1u : 0u));
+        const uint32_t prev = uniform(
+            __hip_atomic_fetch_add(done + r, lane == 0 ? 1u : 0u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT));
         if (prev + 1 == items) {
-            __threadfence();
             const uint32_t base = uniform(d->item_base);
             reduce_request(kind, item_rec + (size_t)base * kBatchRec, base, items, seq, r, lane,
                            result_host + (size_t)r * kBatchRec);
-            // publish: the record reaches host memory before the flag does
-            __threadfence_system();
-            flag_host[r] = seq;
+            // publish: the record's stores are ordered before the flag's
+            __hip_atomic_store(flag_host + r, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         i = begin + uniform(atomicAdd(counter + q, lane == 0 ? 1u : 0u));
     }
 }
 
-// Resets the four work counters and the per-request item counts (a kernel,
-// so that a captured graph holds no memset node).
-__global__ void batch_reset_kernel(unsigned int* counter, unsigned int* done, unsigned int n_done)
+// Copies the staged batch from host-mapped (coherent, uncached) memory into
+// the device arena and zeroes the four work counters and the per-request
+// item counts.  The byte count is the staged header's; every region of the
+// arena is 64-byte aligned, so the copy moves 16-byte words.
+__global__ void __launch_bounds__(256) batch_stage_kernel(const uint8_t* __restrict__ stage,
+                                                          uint8_t* __restrict__ arena, unsigned int* counter,
+                                                          unsigned int* done, unsigned int n_done)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t words = ((const BatchHeader*)stage)->bytes / 16;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint4* __restrict__ src = (const uint4*)stage;
+    uint4* __restrict__ dst = (uint4*)arena;
+    for (uint64_t w = t; w < words; w += stride) dst[w] = src[w];
     if (t < 4) counter[t] = 0u;
-    for (uint32_t j = t; j < n_done; j += gridDim.x * blockDim.x) done[j] = 0u;
+    for (uint64_t j = t; j < n_done; j += stride) done[j] = 0u;
 }
 
 const void* batch_walk_kernel_ptr(int kinds)
@@ -285,6 +323,6 @@ const void* batch_walk_kernel_ptr(int kinds)
     default: return nullptr;
     }
 }
-const void* batch_reset_kernel_ptr() { return reinterpret_cast<const void*>(&batch_reset_kernel); }
+const void* batch_stage_kernel_ptr() { return reinterpret_cast<const void*>(&batch_stage_kernel); }
 
 }  // namespace spm

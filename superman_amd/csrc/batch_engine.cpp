@@ -5,7 +5,8 @@
 // pinned staging buffer, a device arena (header, request descriptors, item
 // table, matrix data), item and result records, the work counters and,
 // optionally, a captured hipGraph of the batch's launch sequence
-//   reset counters -> walk kernel(s).
+//   upload (a kernel that copies the staged batch and zeroes the counters)
+//   -> walk kernel(s).
 // Each request's result is published by the walk kernel itself into
 // host-mapped memory followed by a per-request flag, so progress() hands a
 // result out as soon as its flag shows, without waiting for the rest of the
@@ -89,7 +90,8 @@ struct Slot {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    uint8_t* h_stage = nullptr;
+    uint8_t* h_stage = nullptr;     // host-mapped, coherent: read by the upload kernel
+    uint8_t* dh_stage = nullptr;    // its device address
     uint8_t* d_arena = nullptr;
     double* d_item = nullptr;
     double* h_res = nullptr;        // host-mapped result records [max_requests][8]
@@ -300,9 +302,8 @@ hipError_t enqueue(spm_batch* b, Slot& s)
     unsigned int* done = s.d_counter + 64;
     {
         unsigned int n_done = (unsigned int)b->cfg.max_requests;
-        const int blocks = std::min(256, (b->cfg.max_requests + 255) / 256);
-        void* args[] = {&s.d_counter, &done, &n_done};
-        e = hipLaunchKernel(batch_reset_kernel_ptr(), dim3(blocks), dim3(256), args, 0, s.stream);
+        void* args[] = {&s.dh_stage, &s.d_arena, &s.d_counter, &done, &n_done};
+        e = hipLaunchKernel(batch_stage_kernel_ptr(), dim3(kStageBlocks), dim3(256), args, 0, s.stream);
         if (e != hipSuccess) return e;
     }
     const int kinds[3] = {1, 2, 12};
@@ -390,8 +391,8 @@ size_t stage(spm_batch* b, Slot& s, std::vector<Req*>& rs)
     std::memset(&hd, 0, sizeof hd);
     hd.requests = (uint32_t)rs.size();
     hd.seq = s.seq;
-    hd.desc_off = b->desc_off;
-    hd.item_off = b->item_off;
+    hd.desc_off = (uint32_t)b->desc_off;
+    hd.item_off = (uint32_t)b->item_off;
     BatchRequestDesc* desc = (BatchRequestDesc*)(h + b->desc_off);
     uint32_t* item_req = (uint32_t*)(h + b->item_off);
     size_t data = b->data_off;
@@ -421,6 +422,7 @@ size_t stage(spm_batch* b, Slot& s, std::vector<Req*>& rs)
     hd.items = item;
     hd.range_begin[0] = 0;
     hd.range_end[0] = item;
+    hd.bytes = data;
     std::memcpy(h, &hd, sizeof hd);
     return data;
 }
@@ -446,7 +448,7 @@ int launch(spm_batch* b, Slot& s, std::vector<Req*>& rs, double now)
         return SPM_OK;
     }
     ++s.seq;
-    const size_t bytes = stage(b, s, rs);
+    (void)stage(b, s, rs);  // the upload kernel reads the staged size from the header
     s.reqs = rs;  // staged order = record order
     s.taken.assign(rs.size(), 0);
     DeviceGuard g(b->cfg.device);
@@ -457,8 +459,6 @@ int launch(spm_batch* b, Slot& s, std::vector<Req*>& rs, double now)
     };
     hipError_t e = hipEventRecord(s.ev0, s.stream);
     if (e != hipSuccess) return fail(e, "hipEventRecord");
-    e = hipMemcpyAsync(s.d_arena, s.h_stage, bytes, hipMemcpyHostToDevice, s.stream);
-    if (e != hipSuccess) return fail(e, "arena upload");
     if (b->cfg.use_graph) {
         if (!s.exec && (e = capture(b, s)) != hipSuccess) return fail(e, "graph capture");
         if ((e = hipGraphLaunch(s.exec, s.stream)) != hipSuccess) return fail(e, "hipGraphLaunch");
@@ -765,7 +765,8 @@ extern "C" int spm_batch_create(const spm_batch_config* cfg, spm_batch** out)
             for (Slot& t : b->slots) free_slot(t);
             return SPM_EHIP;
         }
-        ok = hipHostMalloc(&s.h_stage, b->stage_bytes, hipHostMallocDefault) == hipSuccess &&
+        ok = hipHostMalloc(&s.h_stage, b->stage_bytes, hipHostMallocMapped | hipHostMallocCoherent) ==
+                 hipSuccess &&
              hipHostMalloc(&s.h_res, sizeof(double) * kBatchRec * cfg->max_requests,
                            hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
              hipHostMalloc(&s.h_flag, sizeof(unsigned int) * cfg->max_requests,
@@ -779,7 +780,8 @@ extern "C" int spm_batch_create(const spm_batch_config* cfg, spm_batch** out)
         }
         std::memset(s.h_flag, 0, sizeof(unsigned int) * cfg->max_requests);
         if (hipHostGetDevicePointer((void**)&s.dh_res, s.h_res, 0) != hipSuccess ||
-            hipHostGetDevicePointer((void**)&s.dh_flag, s.h_flag, 0) != hipSuccess) {
+            hipHostGetDevicePointer((void**)&s.dh_flag, s.h_flag, 0) != hipSuccess ||
+            hipHostGetDevicePointer((void**)&s.dh_stage, s.h_stage, 0) != hipSuccess) {
             for (Slot& t : b->slots) free_slot(t);
             return SPM_EHIP;
         }
@@ -1012,6 +1014,15 @@ extern "C" const char* spm_batch_last_error(spm_batch* b)
     thread_local std::string copy;
     copy = b->last_error;
     return copy.c_str();
+}
+
+extern "C" int spm_batch_counters(spm_batch* b, uint64_t* late_publish, uint64_t* retried)
+{
+    if (!b) return SPM_EINVAL;
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (late_publish) *late_publish = b->late_publish;
+    if (retried) *retried = b->retried;
+    return SPM_OK;
 }
 
 extern "C" int spm_batch_inject_fault(spm_batch* b, int launches)

@@ -142,7 +142,8 @@ using namespace spm;
 // Plans
 // ---------------------------------------------------------------------------
 struct spm_plan {
-    int kind = 0;  // 1 = dense f64, 2 = dense c128, 3 = exact integer, 4 = generated SpaRyser
+    int kind = 0;  // 1 = dense f64, 2 = dense c128, 3 = exact integer, 4 = generated SpaRyser (integer),
+                   // 5 = generated SpaRyser (real FP64)
     int n = 0;
     int precision = 0;
     int device = 0;
@@ -150,6 +151,8 @@ struct spm_plan {
     int L = 0;
     uint64_t chunks = 0, tasks = 0;
     int slices = 1;
+    int max_grid = 0;      // resident blocks on the device (dense kinds)
+    int partial_cap = 0;   // per-block records allocated in d_partial
     int rec = 4;    // doubles per per-block partial record
     int comps = 1;  // double-double components per record (1 real, 2 complex)
     const void* kern = nullptr;
@@ -189,6 +192,20 @@ static void plan_free(spm_plan* p)
             return SPM_EHIP;                           \
         }                                              \
     } while (0)
+
+// Grid and launch slices of a dense plan for its geometry (L, tasks): as many
+// blocks as the tasks fill, at most the resident ones; very long runs
+// (~> 4 s of VALU work at 64 lanes x 256 CUs x 2.4 GHz) are split into
+// several launches that accumulate into the same per-block records.
+static void dense_grid(spm_plan* p)
+{
+    const int n = p->n;
+    const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
+    p->grid = (int)std::min<uint64_t>(uint64_t(p->max_grid), need_blocks);
+    const double ops = p->precision == SPM_PREC_F32_PRODUCT ? 3.0 * n : p->kind == 2 ? 6.0 * n : 2.0 * n;
+    const double est_s = std::ldexp(1.0, n - 1) * ops / 3.9e13;
+    p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
+}
 
 // Dense plans, real (kind 1) and complex (kind 2).  `a` is row-major n x n,
 // doubles or interleaved (re, im) pairs.
@@ -232,13 +249,8 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
     const int ub = kind == 1 ? dense_f64_unroll_bits(n) : (n >= 3 ? 1 : 0);
     gray_geometry(n, ub, waves, &p->L, &p->chunks, &p->tasks, kind == 1 ? GrayWalk::kReal : GrayWalk::kComplex);
-    const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
-    p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
-    // Split very long runs (~> 4 s of VALU work at 64 lanes x 256 CUs x 2.4 GHz)
-    // into several launches that accumulate into the same per-block records.
-    const double ops = f32p ? 3.0 * n : kind == 1 ? 2.0 * n : 6.0 * n;
-    const double est_s = std::ldexp(1.0, n - 1) * ops / 3.9e13;
-    p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
+    p->max_grid = bpc * cus;
+    dense_grid(p);
     char kn[96];
     if (f32p)
         std::snprintf(kn, sizeof kn, "dense_f32p_kernel<%d, %d>", n, ub);
@@ -261,6 +273,7 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * plane));
     SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * col));
     SPM_TRY(hipMalloc(&p->d_partial, sizeof(double) * p->rec * p->grid));
+    p->partial_cap = p->grid;
     SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 6));
     SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 6, hipHostMallocDefault));
     SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(double) * 2 * plane, hipMemcpyHostToDevice));
@@ -495,9 +508,107 @@ extern "C" int spm_plan_codegen(const int64_t* a, int n, int options, int device
     return SPM_OK;
 }
 
+// Generated SpaRyser plans for real FP64 matrices (kind 5): the same
+// per-block records and final reduction as the dense kernels.
+static std::vector<double> codegen_matrix_real(const double* a, int n, int options)
+{
+    if (options & SPM_CG_REGISTER_EFFICIENT) {
+        std::vector<unsigned char> nz((size_t)n * n);
+        for (size_t e = 0; e < nz.size(); ++e) nz[e] = a[e] != 0.0;
+        std::vector<int> rp, cp;
+        ut_order_mask(nz.data(), n, rp, cp);
+        std::vector<double> b((size_t)n * n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) b[(size_t)rp[i] * n + cp[j]] = a[(size_t)i * n + j];
+        return b;
+    }
+    return std::vector<double>(a, a + (size_t)n * n);
+}
+
+static std::string codegen_real_name(uint64_t h)
+{
+    char nm[40];
+    std::snprintf(nm, sizeof nm, "spm_genf_%016llx", (unsigned long long)h);
+    return nm;
+}
+
+static int real_codegen_args(const double* a, int n, int precision)
+{
+    if (!a || n < kCodegenMinN || n > kMaxN) return SPM_EINVAL;
+    if (precision != SPM_PREC_PLAIN && precision != SPM_PREC_COMPENSATED) return SPM_EINVAL;
+    for (int e = 0; e < n * n; ++e)
+        if (!std::isfinite(a[e])) return SPM_EINVAL;
+    return SPM_OK;
+}
+
+extern "C" int spm_codegen_source_real(const double* a, int n, int options, int precision, char* buf, size_t buflen,
+                                       size_t* needed)
+{
+    const int rc = real_codegen_args(a, n, precision);
+    if (rc != SPM_OK) return rc;
+    const std::vector<double> b = codegen_matrix_real(a, n, options);
+    const std::string src = emit_real_kernel(b, n, precision == SPM_PREC_COMPENSATED,
+                                             codegen_real_name(real_pattern_hash(b.data(), n, options, precision)));
+    if (needed) *needed = src.size() + 1;
+    if (buf && buflen > 0) {
+        const size_t k = std::min(buflen - 1, src.size());
+        std::memcpy(buf, src.data(), k);
+        buf[k] = 0;
+    }
+    return SPM_OK;
+}
+
+extern "C" int spm_plan_codegen_real(const double* a, int n, int options, int precision, int device, spm_plan** out)
+{
+    if (!out) return SPM_EINVAL;
+    *out = nullptr;
+    int rc = real_codegen_args(a, n, precision);
+    if (rc != SPM_OK) return rc;
+    const std::vector<double> b = codegen_matrix_real(a, n, options);
+    int cus = 0;
+    rc = device_cus(device, &cus);
+    if (rc != SPM_OK) return rc;
+    DeviceGuard guard(device);
+    if (!guard.ok()) return SPM_EHIP;
+
+    spm_plan* p = new spm_plan;
+    p->kind = 5;
+    p->n = n;
+    p->precision = precision;
+    p->device = device;
+    p->rec = 4;
+    p->comps = 1;
+    p->hash = real_pattern_hash(b.data(), n, options, precision);
+    p->kname = codegen_real_name(p->hash);
+    rc = codegen_get_kernel(device, p->hash,
+                            emit_real_kernel(b, n, precision == SPM_PREC_COMPENSATED, p->kname), p->kname, &p->fn,
+                            &p->compile_s, &p->cached);
+    if (rc != SPM_OK) {
+        plan_free(p);
+        return rc;
+    }
+    int bpc = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, p->fn, kDenseBlock, 0) != hipSuccess || bpc <= 0) {
+        plan_free(p);
+        return SPM_EHIP;
+    }
+    const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
+    gray_geometry(n, codegen_unroll_bits(), waves, &p->L, &p->chunks, &p->tasks);
+    p->max_grid = bpc * cus;
+    dense_grid(p);
+    SPM_TRY(hipMalloc(&p->d_partial, sizeof(double) * p->rec * p->grid));
+    p->partial_cap = p->grid;
+    SPM_TRY(hipMalloc(&p->d_out, sizeof(double) * 6));
+    SPM_TRY(hipHostMalloc(&p->h_out, sizeof(double) * 6, hipHostMallocDefault));
+    SPM_TRY(hipEventCreate(&p->ev0));
+    SPM_TRY(hipEventCreate(&p->ev1));
+    *out = p;
+    return SPM_OK;
+}
+
 extern "C" int spm_plan_codegen_info(const spm_plan* p, double* compile_seconds, int* cached, uint64_t* hash)
 {
-    if (!p || p->kind != 4) return SPM_EINVAL;
+    if (!p || (p->kind != 4 && p->kind != 5)) return SPM_EINVAL;
     if (compile_seconds) *compile_seconds = p->compile_s;
     if (cached) *cached = p->cached ? 1 : 0;
     if (hash) *hash = p->hash;
@@ -523,8 +634,15 @@ static int dense_run(spm_plan* p, hipStream_t st, spm_result* r)
         uint64_t tb = per * s, te = std::min<uint64_t>(p->tasks, per * (s + 1));
         if (tb >= te) break;
         int acc = s > 0 ? 1 : 0;
-        void* args[] = {&p->d_M, &p->d_x0, &p->d_partial, &tb, &te, &p->chunks, &p->L, &acc};
-        if (hipLaunchKernel(p->kern, dim3(p->grid), dim3(kDenseBlock), args, 0, st) != hipSuccess) return SPM_EHIP;
+        if (p->kind == 5) {
+            void* args[] = {&p->d_partial, &tb, &te, &p->chunks, &p->L, &acc};
+            if (hipModuleLaunchKernel(p->fn, p->grid, 1, 1, kDenseBlock, 1, 1, 0, st, args, nullptr) != hipSuccess)
+                return SPM_EHIP;
+        } else {
+            void* args[] = {&p->d_M, &p->d_x0, &p->d_partial, &tb, &te, &p->chunks, &p->L, &acc};
+            if (hipLaunchKernel(p->kern, dim3(p->grid), dim3(kDenseBlock), args, 0, st) != hipSuccess)
+                return SPM_EHIP;
+        }
         ++launches;
     }
     {
@@ -604,7 +722,7 @@ static int int_run(spm_plan* p, hipStream_t st, spm_result* r)
 extern "C" int spm_plan_run(spm_plan* p, void* stream, spm_result* out)
 {
     if (!p || !out) return SPM_EINVAL;
-    if (p->kind == 1 || p->kind == 2) return dense_run(p, (hipStream_t)stream, out);
+    if (p->kind == 1 || p->kind == 2 || p->kind == 5) return dense_run(p, (hipStream_t)stream, out);
     if (p->kind == 3 || p->kind == 4) return int_run(p, (hipStream_t)stream, out);
     return SPM_EINVAL;
 }
@@ -626,6 +744,29 @@ extern "C" int spm_plan_info(const spm_plan* p, int* grid, int* block, int* log2
 }
 
 extern "C" const char* spm_plan_kernel_name(const spm_plan* p) { return p ? p->kname.c_str() : ""; }
+
+extern "C" int spm_plan_set_log2_chunk(spm_plan* p, int L)
+{
+    if (!p || (p->kind != 1 && p->kind != 2)) return SPM_EINVAL;
+    const int n = p->n, B = n - 1;
+    const int ub = p->kind == 1 ? dense_f64_unroll_bits(n) : (n >= 3 ? 1 : 0);
+    const int lo = ub > 0 ? ub + 1 : (B >= 1 ? 1 : 0);
+    if (L < lo || L > B) return SPM_EINVAL;
+    p->L = L;
+    p->chunks = uint64_t(1) << (B - L);
+    p->tasks = (p->chunks + kWaveLanes - 1) / kWaveLanes;
+    dense_grid(p);
+    if (p->grid > p->partial_cap) {
+        DeviceGuard guard(p->device);
+        if (!guard.ok()) return SPM_EHIP;
+        double* np = nullptr;
+        if (hipMalloc(&np, sizeof(double) * p->rec * p->grid) != hipSuccess) return SPM_ENOMEM;
+        if (p->d_partial) (void)hipFree(p->d_partial);
+        p->d_partial = np;
+        p->partial_cap = p->grid;
+    }
+    return SPM_OK;
+}
 
 static int dense_oneshot(int kind, const double* a, int n, int precision, int device, void* stream,
                          spm_result* out)
@@ -671,6 +812,182 @@ extern "C" int spm_sparse_codegen(const int64_t* a, int n, int options, int devi
     plan_free(p);
     if (st == SPM_OK) out->seconds = now_seconds() - t0;
     else out->status = st;
+    return st;
+}
+
+extern "C" int spm_sparse_codegen_real(const double* a, int n, int options, int precision, int device, void* stream,
+                                       spm_result* out)
+{
+    if (!out) return SPM_EINVAL;
+    const double t0 = now_seconds();
+    spm_plan* p = nullptr;
+    int st = spm_plan_codegen_real(a, n, options, precision, device, &p);
+    if (st != SPM_OK) {
+        std::memset(out, 0, sizeof *out);
+        out->status = st;
+        return st;
+    }
+    st = dense_run(p, (hipStream_t)stream, out);
+    plan_free(p);
+    if (st == SPM_OK) out->seconds = now_seconds() - t0;
+    else out->status = st;
+    return st;
+}
+
+// ---------------------------------------------------------------------------
+// Automatic path choice for one matrix.  The reference's rule
+// (AlgorithmSelector.h:52-67) picks between its dense kernel and code
+// generation by nnz per row and component size, with costs of its own GPUs
+// and a recompile of the whole program per pattern.  Here every candidate's
+// time is estimated from rates measured on MI355X (per Gray subset) plus the
+// hiprtc compile where a generated kernel is not yet in the module cache,
+// and the cheapest runs:
+//   SPM_PATH_INT        integer matrix: exact walk (compressed columns for
+//                       sparse ones, else FP64 or int32 walk)
+//   SPM_PATH_GEN_INT    integer matrix: exact generated kernel
+//   SPM_PATH_SCALED_INT rows s_i * integer vectors: exact integer permanent
+//                       times prod s_i (double-double), one rounding
+//   SPM_PATH_GEN_REAL   real matrix: generated FP64 kernel
+//   SPM_PATH_DENSE      real matrix: dense FP64 walk
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Shape {
+    int n = 0;
+    size_t nnz = 0;
+    bool integral = false;  // every entry an integer of magnitude <= 2^30
+};
+
+Shape shape_of(const double* a, int n)
+{
+    Shape s;
+    s.n = n;
+    s.integral = true;
+    for (size_t e = 0; e < (size_t)n * n; ++e) {
+        const double v = a[e];
+        s.nnz += v != 0.0 ? 1 : 0;
+        if (!(std::fabs(v) <= 1073741824.0) || v != std::floor(v)) s.integral = false;
+    }
+    return s;
+}
+
+double subsets(int n) { return std::ldexp(1.0, n - 1); }
+double per_row(const Shape& s) { return double(s.nnz) / s.n; }
+
+// Seconds, from the kernels' measured rates on MI355X (fixed launch costs
+// of a few tens of microseconds included).
+double est_dense(const Shape& s) { return 3e-5 + subsets(s.n) * 2.0 * s.n / 3.5e13; }
+double est_int_walk(const Shape& s)
+{
+    if (s.n >= 26 && 4 * s.nnz <= (size_t)s.n * s.n)  // compressed-column walk
+        return 3e-5 + subsets(s.n) * 1.03e-12 * per_row(s) / 4.0;
+    return 3e-5 + subsets(s.n) * s.n * 6.4e-14;     // FP64 walk
+}
+double est_gen_int(const Shape& s) { return 3e-5 + subsets(s.n) * 2.6e-13 * per_row(s) / 4.0; }
+double est_gen_real(const Shape& s) { return 3e-5 + subsets(s.n) * (per_row(s) + 14.0) * 3.8e-14; }
+double est_compile(const Shape& s) { return 0.5 + 1.6e-4 * s.n * double(s.nnz); }
+
+}  // namespace
+
+static int choose_path(const double* a, int n, bool gen_cached, int* path)
+{
+    const Shape s = shape_of(a, n);
+    const bool can_gen = n >= kCodegenMinN;
+    const double compile = gen_cached ? 0.0 : est_compile(s);
+    if (s.integral) {
+        *path = can_gen && est_gen_int(s) + compile < est_int_walk(s) ? SPM_PATH_GEN_INT : SPM_PATH_INT;
+        return SPM_OK;
+    }
+    std::vector<int64_t> q;
+    double sh = 0, sl = 0;
+    if (row_scaled_integer(a, n, q, &sh, &sl)) {
+        *path = SPM_PATH_SCALED_INT;
+        return SPM_OK;
+    }
+    *path = can_gen && est_gen_real(s) + compile < est_dense(s) ? SPM_PATH_GEN_REAL : SPM_PATH_DENSE;
+    return SPM_OK;
+}
+
+extern "C" int spm_choose_path(const double* a, int n, int* path)
+{
+    if (!a || !path || n < 1 || n > kMaxN) return SPM_EINVAL;
+    for (size_t e = 0; e < (size_t)n * n; ++e)
+        if (!std::isfinite(a[e])) return SPM_EINVAL;
+    return choose_path(a, n, false, path);
+}
+
+extern "C" int spm_path_estimate(const double* a, int n, int path, double* seconds)
+{
+    if (!a || !seconds || n < 1 || n > kMaxN) return SPM_EINVAL;
+    const Shape s = shape_of(a, n);
+    switch (path) {
+    case SPM_PATH_INT: *seconds = est_int_walk(s); break;
+    case SPM_PATH_GEN_INT: *seconds = est_gen_int(s) + est_compile(s); break;
+    case SPM_PATH_SCALED_INT: *seconds = est_int_walk(s); break;
+    case SPM_PATH_GEN_REAL: *seconds = est_gen_real(s) + est_compile(s); break;
+    case SPM_PATH_DENSE: *seconds = est_dense(s); break;
+    default: return SPM_EINVAL;
+    }
+    return SPM_OK;
+}
+
+// Integer matrix q by the chosen integer path; SPM_ERANGE when the exact
+// range does not hold.
+static int run_int_path(const int64_t* q, int n, bool gen, int device, void* stream, spm_result* out)
+{
+    if (gen) return spm_sparse_codegen(q, n, SPM_CG_REGISTER_EFFICIENT, device, stream, out);
+    return spm_int_exact(q, n, device, stream, out);
+}
+
+extern "C" int spm_permanent_auto(const double* a, int n, int precision, int device, void* stream, spm_result* out,
+                                  int* path_taken)
+{
+    if (!out) return SPM_EINVAL;
+    std::memset(out, 0, sizeof *out);
+    if (!a || n < 1 || n > kMaxN) return out->status = SPM_EINVAL;
+    if (precision != SPM_PREC_PLAIN && precision != SPM_PREC_COMPENSATED) return out->status = SPM_EINVAL;
+    for (size_t e = 0; e < (size_t)n * n; ++e)
+        if (!std::isfinite(a[e])) return out->status = SPM_EINVAL;
+    const double t0 = now_seconds();
+    int path = SPM_PATH_DENSE;
+    int st = choose_path(a, n, false, &path);
+    if (st != SPM_OK) return out->status = st;
+    if (path == SPM_PATH_INT || path == SPM_PATH_GEN_INT) {
+        std::vector<int64_t> q((size_t)n * n);
+        for (size_t e = 0; e < q.size(); ++e) q[e] = (int64_t)a[e];
+        st = run_int_path(q.data(), n, path == SPM_PATH_GEN_INT, device, stream, out);
+        if (st != SPM_ERANGE) {
+            if (path_taken) *path_taken = path;
+            if (st == SPM_OK) out->seconds = now_seconds() - t0;
+            return st;
+        }
+        path = SPM_PATH_DENSE;  // beyond the exact range: FP64
+    } else if (path == SPM_PATH_SCALED_INT) {
+        std::vector<int64_t> q;
+        double sh = 0, sl = 0;
+        row_scaled_integer(a, n, q, &sh, &sl);
+        std::vector<double> qd(q.begin(), q.end());
+        int qp = SPM_PATH_INT;
+        choose_path(qd.data(), n, false, &qp);
+        spm_result t;
+        st = run_int_path(q.data(), n, qp == SPM_PATH_GEN_INT, device, stream, &t);
+        if (st == SPM_OK) {
+            scaled_int_result(sh, sl, t, out);
+            if (path_taken) *path_taken = path;
+            out->seconds = now_seconds() - t0;
+            return SPM_OK;
+        }
+        if (st != SPM_ERANGE) return out->status = st;
+        Shape sp = shape_of(a, n);
+        path = n >= kCodegenMinN && est_gen_real(sp) + est_compile(sp) < est_dense(sp) ? SPM_PATH_GEN_REAL
+                                                                                      : SPM_PATH_DENSE;
+    }
+    if (path == SPM_PATH_GEN_REAL)
+        st = spm_sparse_codegen_real(a, n, SPM_CG_REGISTER_EFFICIENT, precision, device, stream, out);
+    else
+        st = spm_dense_f64(a, n, precision, device, stream, out);
+    if (path_taken) *path_taken = path;
+    if (st == SPM_OK) out->seconds = now_seconds() - t0;
     return st;
 }
 

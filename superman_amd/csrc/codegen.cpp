@@ -122,6 +122,13 @@ int gen_waves_per_simd(int n)
     return 0;
 }
 
+// Real kernels hold x as FP64 (2 VGPRs per row); 0 = the compiler's choice.
+int gen_real_waves_per_simd(int n)
+{
+    (void)n;
+    return 0;
+}
+
 std::string lit(int64_t v)
 {
     return std::to_string(v);  // integers are exact in decimal
@@ -279,6 +286,268 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
          "    }\n"
          "}\n";
     return o.str();
+}
+
+// ---------------------------------------------------------------------------
+// Source emitter (real FP64 entries).
+//
+// The reference's generated kernels compute in FP64 with decimal literals
+// printed by std::to_string (6 decimals, UTOrderedKernelCodeGen.cuh:292-295),
+// exact only for dyadic values with few digits.  Here every literal is the
+// hex-float spelling of the matrix entry (%a), so the kernel adds exactly the
+// entry's binary64 value.  Exact zeros of x_i(S) need exact cancellation, so
+// generic real matrices get no zero-count skip (integer-valued and row-scaled
+// integer matrices take the exact integer kernels instead); what they get is
+// a product that does not recompute the rows a flip cannot have changed:
+//
+//   R(c)  = the largest row any of columns 0..c touches (rows in UT order
+//           are numbered as columns first touch them, so R grows slowly);
+//   S_j   = prod_{r > R(j)} x_r, j = 0..J-1 (cached "suffix" products);
+//   term  = (prod_{r <= R(0)} x_r) * S_0.
+//
+// Flipping column c changes rows <= R(c) only: S_j for j >= c stays valid,
+// S_{c-1} .. S_0 are rebuilt from S_c (one level each), and a column c >= J
+// rebuilds S_{J-1} from its rows first.  J is chosen per pattern to minimise
+// the expected multiplies per step (column c flips every 2^(c+1) steps).
+// ---------------------------------------------------------------------------
+namespace {
+
+std::string hexlit(double v)
+{
+    char buf[48];
+    std::snprintf(buf, sizeof buf, "%a", v);
+    return buf;
+}
+
+struct RealLevels {
+    std::vector<int> R;  // R[c], c = 0..n-2 (-1: no row yet)
+    int J = 1;
+};
+
+RealLevels real_levels(const std::vector<double>& b, int n)
+{
+    RealLevels lv;
+    lv.R.assign(std::max(1, n - 1), -1);
+    int cur = -1;
+    for (int c = 0; c + 1 < n; ++c) {
+        for (int r = 0; r < n; ++r)
+            if (b[(size_t)r * n + c] != 0.0) cur = std::max(cur, r);
+        lv.R[c] = cur;
+    }
+    if (n < 2) return lv;
+    // expected multiplies per step for J levels (column c flips with
+    // probability 2^-(c+1) per step; columns >= J rebuild the top level)
+    auto cost = [&](int J) {
+        const double R0 = lv.R[0];
+        double e = R0 + 1.0;
+        for (int c = 1; c < J && c < n - 1; ++c) e += std::ldexp(lv.R[c] - R0 + c, -(c + 1));
+        const int top = lv.R[std::min(J, n - 1) - 1];
+        e += std::ldexp((n - 1 - top) + (top - R0) + J - 1, -J);
+        return e;
+    };
+    double best = 1e300;
+    for (int J = 1; J <= std::min(8, n - 1); ++J) {
+        const double e = cost(J);
+        if (e < best - 1e-9) {
+            best = e;
+            lv.J = J;
+        }
+    }
+    return lv;
+}
+
+// Product of x_lo .. x_hi (inclusive) as an expression in up to four
+// independent chains ("1.0" when empty).
+std::string prod_expr(int lo, int hi)
+{
+    if (hi < lo) return "1.0";
+    const int k = hi - lo + 1;
+    const int chains = k >= 8 ? 4 : k >= 4 ? 2 : 1;
+    std::vector<std::string> ch(chains);
+    for (int i = 0; i < k; ++i) {
+        std::string& c = ch[i % chains];
+        c += (c.empty() ? "" : " * ") + std::string("x") + std::to_string(lo + i);
+    }
+    std::string e;
+    for (int i = 0; i < chains; ++i) e += (i ? " * (" : "(") + ch[i] + ")";
+    return e;
+}
+
+void emit_real_flip(std::ostringstream& o, const std::vector<double>& b, int n, int c, int sgn, const char* ind)
+{
+    for (int r = 0; r < n; ++r) {
+        const double d = b[(size_t)r * n + c];
+        if (d == 0.0) continue;
+        if (sgn == 0)
+            o << ind << "x" << r << " = __builtin_fma(sd, " << hexlit(d) << ", x" << r << ");\n";
+        else
+            o << ind << "x" << r << " += " << hexlit(sgn > 0 ? d : -d) << ";\n";
+    }
+}
+
+// S_{from-1} .. S_0 from S_from; from == J rebuilds S_{J-1} from its rows.
+void emit_real_levels(std::ostringstream& o, const RealLevels& lv, int n, int from, const char* ind)
+{
+    const int J = lv.J;
+    if (from >= J) {
+        o << ind << "S" << J - 1 << " = " << prod_expr(lv.R[J - 1] + 1, n - 1) << ";\n";
+        from = J - 1;
+    }
+    for (int j = from - 1; j >= 0; --j)
+        o << ind << "S" << j << " = " << prod_expr(lv.R[j] + 1, lv.R[j + 1]) << " * S" << j + 1 << ";\n";
+}
+
+void emit_real_term(std::ostringstream& o, const RealLevels& lv, bool negative, const char* ind)
+{
+    o << ind << "{ const double P = " << prod_expr(0, lv.R[0]) << "; blk = __builtin_fma(" << (negative ? "-P" : "P")
+      << ", S0, blk); ba = __builtin_fma(__builtin_fabs(P), __builtin_fabs(S0), ba); }\n";
+}
+
+}  // namespace
+
+std::string emit_real_kernel(const std::vector<double>& b, int n, bool compensated, const std::string& name)
+{
+    const int U = kGenU;
+    const RealLevels lv = real_levels(b, n);
+    std::vector<double> x0(n);
+    for (int i = 0; i < n; ++i) {  // x(empty set) with the left-to-right row sum (dpSingleGPU.cuh:62-74)
+        double rs = 0.0;
+        for (int j = 0; j < n; ++j) rs += b[(size_t)i * n + j];
+        x0[i] = b[(size_t)i * n + n - 1] - rs / 2;
+    }
+    std::ostringstream o;
+    o << "// generated: SpaRyser kernel for one " << n << "x" << n << " real matrix (FP64, "
+      << (compensated ? "compensated" : "plain") << " sums, " << lv.J << " suffix-product levels)\n";
+    o << "static __device__ __forceinline__ void ts(double a, double b, double& s, double& e)\n"
+         "{ s = a + b; const double bb = s - a; e = (a - (s - bb)) + (b - bb); }\n"
+         "static __device__ __forceinline__ void ddadd(double& ah, double& al, double bh, double bl)\n"
+         "{ double s, e; ts(ah, bh, s, e); e += al + bl; ah = s + e; al = e - (ah - s); }\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(256)";
+    if (const int w = gen_real_waves_per_simd(n)) o << " __attribute__((amdgpu_waves_per_eu(" << w << ")))";
+    o << "\n" << name
+      << "(double* __restrict__ partial, unsigned long long task_begin,\n"
+         "    unsigned long long task_end, unsigned long long num_chunks, int L, int accumulate)\n{\n";
+    o << "    __shared__ double red[12];\n"
+         "    const unsigned lane = threadIdx.x & 63u;\n"
+         "    const unsigned long long wave = (unsigned long long)blockIdx.x * 4 + (threadIdx.x >> 6);\n"
+         "    const unsigned long long nwave = (unsigned long long)gridDim.x * 4;\n"
+         "    double a_hi = 0.0, a_lo = 0.0, a_abs = 0.0;\n"
+         "    for (unsigned long long task = task_begin + wave; task < task_end; task += nwave) {\n"
+         "        const unsigned long long h = task * 64 + lane;\n";
+    o << "        double ";
+    for (int i = 0; i < n; ++i) o << "x" << i << " = " << hexlit(x0[i]) << (i + 1 < n ? ", " : ";\n");
+    o << "        const unsigned long long k0 = h << L;\n"
+         "        const unsigned long long g0 = k0 ^ (k0 >> 1);\n";
+    // chunk-start rebuild in ascending column order (DenseKernelDefinitions.cuh:230-240)
+    for (int j = 0; j < n - 1; ++j) {
+        bool any = false;
+        for (int r = 0; r < n; ++r) any = any || b[(size_t)r * n + j] != 0.0;
+        if (!any) continue;
+        o << "        { const double bj = ((((g0 >> " << j << ") & 1ull) != 0ull) && (" << j
+          << " >= L - 1)) ? 1.0 : 0.0;";
+        for (int r = 0; r < n; ++r) {
+            const double d = b[(size_t)r * n + j];
+            if (d != 0.0) o << " x" << r << " = __builtin_fma(bj, " << hexlit(d) << ", x" << r << ");";
+        }
+        o << " }\n";
+    }
+    o << "        double ";
+    for (int j = 0; j < lv.J; ++j) o << "S" << j << (j + 1 < lv.J ? ", " : ";\n");
+    emit_real_levels(o, lv, n, lv.J, "        ");
+    o << "        const double sd = (h & 1ull) ? -1.0 : 1.0;  // direction of column L-1\n"
+         "        double t_hi = 0.0, t_lo = 0.0, t_abs = 0.0, blk = 0.0, ba = 0.0;\n";
+    o << "        const unsigned nblk = 1u << (L - " << U << ");\n"
+         "        const unsigned half = nblk >> 1;\n";
+    const std::string fold = compensated
+        ? "{ double s_, e_; ts(t_hi, blk, s_, e_); t_hi = s_; t_lo += e_; t_abs += ba; blk = 0.0; ba = 0.0; }\n"
+        : "{ t_hi += blk; t_abs += ba; blk = 0.0; ba = 0.0; }\n";
+    auto flip = [&](int c, int sgn, const char* ind) {
+        emit_real_flip(o, b, n, c, sgn, ind);
+        if (c > 0) emit_real_levels(o, lv, n, c, ind);
+    };
+    auto block_tail = [&](const char* ind) {
+        for (int j = 1; j < (1 << U); ++j) {
+            const int c = __builtin_ctz(j);
+            if (c + 1 < U) {
+                flip(c, ((j >> (c + 1)) & 1) ? -1 : 1, ind);
+            } else {
+                o << ind << "if (m & 1u) {\n";
+                flip(c, -1, (std::string(ind) + "    ").c_str());
+                o << ind << "} else {\n";
+                flip(c, 1, (std::string(ind) + "    ").c_str());
+                o << ind << "}\n";
+            }
+            emit_real_term(o, lv, (j & 1) != 0, ind);
+        }
+        o << ind << fold;
+    };
+    o << "        {\n            const unsigned m = 0;\n";
+    emit_real_term(o, lv, false, "            ");
+    block_tail("            ");
+    o << "        }\n";
+    o << "        for (unsigned m = 1; m < nblk; ++m) {\n"
+         "            const int tz = __builtin_ctz(m);\n"
+         "            const int c = tz + " << U << ";\n"
+         "            const int dir = (m == half) ? 0 : (((m >> (tz + 1)) & 1u) ? -1 : 1);\n"
+         "            switch (c) {\n";
+    for (int c = U; c < n - 1; ++c) {
+        o << "            case " << c << ":\n"
+             "                if (dir == 1) {\n";
+        flip(c, 1, "                    ");
+        o << "                } else if (dir == -1) {\n";
+        flip(c, -1, "                    ");
+        o << "                } else {\n";
+        flip(c, 0, "                    ");
+        o << "                }\n                break;\n";
+    }
+    o << "            default: break;\n            }\n";
+    emit_real_term(o, lv, false, "            ");
+    block_tail("            ");
+    o << "        }\n";
+    o << "        if (h < num_chunks) { ddadd(a_hi, a_lo, t_hi, t_lo); a_abs += t_abs; }\n"
+         "    }\n"
+         "    for (int off = 32; off >= 1; off >>= 1) {\n"
+         "        const double oh = __shfl_xor(a_hi, off, 64), ol = __shfl_xor(a_lo, off, 64);\n"
+         "        const double oa = __shfl_xor(a_abs, off, 64);\n"
+         "        ddadd(a_hi, a_lo, oh, ol);\n"
+         "        a_abs += oa;\n"
+         "    }\n"
+         "    const int w = threadIdx.x >> 6;\n"
+         "    if (lane == 0) { red[3 * w] = a_hi; red[3 * w + 1] = a_lo; red[3 * w + 2] = a_abs; }\n"
+         "    __syncthreads();\n"
+         "    if (threadIdx.x == 0) {\n"
+         "        double hh = red[0], ll = red[1], aa = red[2];\n"
+         "        for (int i = 1; i < 4; ++i) { ddadd(hh, ll, red[3 * i], red[3 * i + 1]); aa += red[3 * i + 2]; }\n"
+         "        double* out = partial + 4 * blockIdx.x;\n"
+         "        if (accumulate) { ddadd(hh, ll, out[0], out[1]); aa += out[2]; }\n"
+         "        out[0] = hh; out[1] = ll; out[2] = aa; out[3] = 0.0;\n"
+         "    }\n"
+         "}\n";
+    return o.str();
+}
+
+int real_levels_count(const std::vector<double>& b, int n) { return real_levels(b, n).J; }
+
+// FNV-1a over the values' bit patterns and the generator options.
+uint64_t real_pattern_hash(const double* b, int n, int options, int precision)
+{
+    uint64_t h = 1469598103934665603ull ^ 0x5a5a5a5a5a5a5a5aull;
+    auto mix = [&](uint64_t v) {
+        for (int k = 0; k < 8; ++k) {
+            h ^= (v >> (8 * k)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    };
+    mix((uint64_t)n);
+    mix((uint64_t)options);
+    mix((uint64_t)precision);
+    mix((uint64_t)kGenU);
+    for (int e = 0; e < n * n; ++e) {
+        uint64_t w;
+        std::memcpy(&w, b + e, 8);
+        mix(w);
+    }
+    return h;
 }
 
 int codegen_unroll_bits() { return kGenU; }

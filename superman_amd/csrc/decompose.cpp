@@ -496,17 +496,6 @@ extern "C" int spm_decomp_destroy(spm_decomp* d)
 
 // ---- the permanent through the decomposition ----------------------------------
 
-namespace {
-
-bool integral(const Mat& m)
-{
-    for (double v : m.a)
-        if (v != std::floor(v) || std::fabs(v) > 1073741824.0) return false;
-    return true;
-}
-
-}  // namespace
-
 extern "C" int spm_permanent_decomposed(const double* a, int n, int options, int precision, int device, void* stream,
                                         spm_result* out)
 {
@@ -528,7 +517,8 @@ extern "C" int spm_permanent_decomposed(const double* a, int n, int options, int
         return SPM_OK;
     }
     // Leaves: exact for integer matrices (while the exact range allows),
-    // else FP64 with the requested accumulation.
+    // else FP64 with the requested accumulation; a component's sum stays
+    // exact while every leaf of the whole matrix is.
     bool exact = true;
     std::vector<__int128> comp_exact(d->components, 0);
     std::vector<double> comp_val(d->components, 0.0), comp_err(d->components, 0.0);
@@ -540,25 +530,16 @@ extern "C" int spm_permanent_decomposed(const double* a, int n, int options, int
             out->status = SPM_EINVAL;  // DecomposePerman.h:185-188
             return SPM_EINVAL;
         }
+        // each leaf by the automatic path choice (exact integer walks or
+        // generated kernels, row-scaled integers, generated FP64 kernels,
+        // the dense FP64 walk)
         spm_result r;
-        st = SPM_ERANGE;
-        if (integral(m)) {
-            std::vector<int64_t> ai(m.a.size());
-            for (size_t e = 0; e < m.a.size(); ++e) ai[e] = (int64_t)m.a[e];
-            st = spm_int_exact(ai.data(), m.n, device, stream, &r);
-            if (st != SPM_OK && st != SPM_ERANGE) {
-                out->status = st;
-                return st;
-            }
-        }
+        st = spm_permanent_auto(m.a.data(), m.n, precision, device, stream, &r, nullptr);
         if (st != SPM_OK) {
-            exact = false;
-            st = spm_dense_f64(m.a.data(), m.n, precision, device, stream, &r);
-            if (st != SPM_OK) {
-                out->status = st;
-                return st;
-            }
+            out->status = st;
+            return st;
         }
+        if (!r.is_exact) exact = false;
         const int c = d->leaf_comp[k];
         if (r.is_exact) {
             const __int128 v = ((__int128)r.exact_hi << 64) | (unsigned __int128)(uint64_t)r.exact_lo;

@@ -151,6 +151,69 @@ bool int_sparse_preferred(const int64_t* a, int n)
     return 4 * nnz <= (size_t)n * n;
 }
 
+// Rows that are scalar multiples of integer vectors: perm(A) = prod_i s_i *
+// perm(Q) with a_ij = s_i q_ij exactly (multilinearity in the rows).  s_i
+// is the row's smallest absolute entry; every q_ij = a_ij / s_i must be an
+// integer of magnitude <= 2^20 that reproduces a_ij exactly, and the
+// product of the s_i (kept as a double-double) must stay normal.  False for
+// any other row (or an all-zero row, whose permanent the caller gets
+// anyway), and for integer matrices (every s_i would be 1 only by accident
+// of their entries; those take the exact path directly).
+bool row_scaled_integer(const double* a, int n, std::vector<int64_t>& q, double* scale_hi, double* scale_lo)
+{
+    q.assign((size_t)n * n, 0);
+    double sh = 1.0, sl = 0.0;
+    bool all_one = true;
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const double v = std::fabs(a[(size_t)i * n + j]);
+            if (!std::isfinite(v)) return false;
+            if (v != 0.0 && (s == 0.0 || v < s)) s = v;
+        }
+        if (s == 0.0) return false;
+        for (int j = 0; j < n; ++j) {
+            const double v = a[(size_t)i * n + j];
+            if (v == 0.0) continue;
+            const double t = v / s;
+            const double r = std::nearbyint(t);
+            if (r != t || std::fabs(r) > 1048576.0 || r * s != v) return false;
+            q[(size_t)i * n + j] = (int64_t)r;
+        }
+        all_one = all_one && s == 1.0;
+        // (sh, sl) *= s exactly to double-double precision (TwoProd by FMA)
+        const double p = sh * s;
+        const double e = std::fma(sh, s, -p) + sl * s;
+        sh = p + e;
+        sl = e - (sh - p);
+    }
+    if (all_one || !std::isnormal(sh)) return false;
+    *scale_hi = sh;
+    *scale_lo = sl;
+    return true;
+}
+
+// value = (scale_hi + scale_lo) * T for an exact 128-bit T, rounded once
+// from double-double; fills the result as a (non-integer) real value.
+void scaled_int_result(double scale_hi, double scale_lo, const spm_result& t, spm_result* r)
+{
+    const __int128 T = ((__int128)t.exact_hi << 64) | (unsigned __int128)(uint64_t)t.exact_lo;
+    const double th = (double)T;
+    const double tl = (double)(T - (__int128)th);
+    const double p = scale_hi * th;
+    const double e = std::fma(scale_hi, th, -p) + (scale_hi * tl + scale_lo * th);
+    const double hi = p + e;
+    const double lo = e - (hi - p);
+    *r = t;
+    r->is_exact = 0;
+    r->exact_lo = r->exact_hi = 0;
+    r->re = hi;
+    r->hi_re = hi;
+    r->lo_re = lo;
+    r->abs_sum = 0.0;
+    r->err_est = std::ldexp(std::fabs(hi), -52);  // the final rounding and the double-double scale
+}
+
 // True when every absolute row sum is below 2^int_small_bits(n), so that
 // the FP64 integer walk (int_small_kernel, batch kind 4) is exact.
 bool int_small_ok(const int64_t* a, int n)

@@ -29,7 +29,8 @@ inline uint64_t batch_tag(uint32_t seq, uint32_t idx)
 
 // Check word of a result record: an FNV-1a style mix of its other words'
 // bit patterns, so that a reader can tell a whole record from one whose
-// stores have not all arrived.
+// stores have not all arrived (the flag's release store orders them; the
+// host checks anyway).
 #ifdef __HIPCC__
 __host__ __device__
 #endif
@@ -45,12 +46,13 @@ constexpr int kBatchAlign = 64;     // byte alignment of every arena region
 struct BatchHeader {                // 64 bytes at arena offset 0
     uint32_t requests;
     uint32_t items;
-    uint64_t desc_off;              // BatchRequestDesc[requests]
-    uint64_t item_off;              // uint32_t[items]: item -> request
+    uint32_t desc_off;              // BatchRequestDesc[requests]
+    uint32_t item_off;              // uint32_t[items]: item -> request
     uint32_t range_begin[4];        // item range walked by kernel q: q = 0 all kinds
     uint32_t range_end[4];          // (fused kernel); q = 1 real, 2 complex, 3 integer
     uint32_t seq;                   // batch number, written to a request's flag when done
     uint32_t pad;
+    uint64_t bytes;                 // staged bytes: the upload kernel copies [0, bytes)
 };
 
 struct BatchRequestDesc {           // 64 bytes
@@ -76,8 +78,13 @@ static_assert(sizeof(BatchRequestDesc) == 64, "descriptor layout");
 //          is published there, then its flag is set to the header's seq.
 //          `kinds` = 15 (every kind in one kernel), 1 (real), 2 (complex)
 //          or 12 (both integer walks)
-//   reset: (unsigned int* counter, unsigned int* done, unsigned int n_done)
+//   stage: (const uint8_t* stage /* host-mapped, coherent */, uint8_t* arena,
+//           unsigned int* counter, unsigned int* done, unsigned int n_done)
+//          copies the staged batch (header.bytes bytes, a multiple of 16)
+//          into the device arena and zeroes the work counters and the
+//          per-request item counts; launched with kStageBlocks x 256.
 const void* batch_walk_kernel_ptr(int kinds);
-const void* batch_reset_kernel_ptr();
+const void* batch_stage_kernel_ptr();
+constexpr int kStageBlocks = 128;
 
 }  // namespace spm

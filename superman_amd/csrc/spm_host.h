@@ -60,6 +60,8 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob);
 std::vector<int64_t> ascending_col_sort(const int64_t* a, int n);
 bool int_sparse_preferred(const int64_t* a, int n);
 bool int_small_ok(const int64_t* a, int n);
+bool row_scaled_integer(const double* a, int n, std::vector<int64_t>& q, double* scale_hi, double* scale_lo);
+void scaled_int_result(double scale_hi, double scale_lo, const spm_result& t, spm_result* r);
 void int_small_planes(const int64_t* a, int n, double* planes);
 void dense_fill_result(int n, bool complex_kind, bool compensated, const double* rec, spm_result* r);
 int f32prod_scale_exponent(int n);
@@ -73,6 +75,9 @@ void ut_order(const int64_t* a, int n, std::vector<int>& rowPerm, std::vector<in
 int codegen_compile_only(const std::string& src, const std::string& name, double* seconds, size_t* code_bytes);
 std::vector<int64_t> apply_perm(const int64_t* a, int n, const std::vector<int>& rp, const std::vector<int>& cp);
 std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::string& name);
+std::string emit_real_kernel(const std::vector<double>& b, int n, bool compensated, const std::string& name);
+int real_levels_count(const std::vector<double>& b, int n);
+uint64_t real_pattern_hash(const double* b, int n, int options, int precision);
 int codegen_unroll_bits();
 uint64_t pattern_hash(const int64_t* b, int n, int options);
 int codegen_get_kernel(int device, uint64_t hash, const std::string& src, const std::string& name, hipFunction_t* fn,

@@ -4,7 +4,11 @@ Bars: real and complex within 1e-10 relative of the double-double oracle
 (and within the result's own error estimate), integers bit-exact against the
 int128 oracle; a request's result is bit-identical alone, in a mixed batch,
 eager or graph-launched, fused or per-kind, and served under Poisson
-arrivals."""
+arrivals.  Every engine here (but the one that injects failing checks) ends
+with late_publish == 0 and retried == 0: no result arrived late and none was
+run again."""
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -44,9 +48,17 @@ def check_vs_oracle(kind, a, p):
         assert abs(p.value - ref) <= p.err_est
 
 
+@contextlib.contextmanager
+def checked(**kw):
+    """A BatchEngine whose late-publish and re-run counters must stay 0."""
+    with BatchEngine(**kw) as eng:
+        yield eng
+        assert eng.counters() == {"late_publish": 0, "retried": 0}, eng.last_error()
+
+
 @pytest.fixture(scope="module")
 def engine():
-    with BatchEngine() as eng:
+    with checked() as eng:
         yield eng
 
 
@@ -124,9 +136,9 @@ def test_alone_equals_in_mix(engine):
 
 def test_graph_replays_equal_eager():
     mix = _mix(60, 12)
-    with BatchEngine(use_graph=False) as eng:
+    with checked(use_graph=False) as eng:
         eager = eng.run(mix)
-    with BatchEngine(use_graph=True) as eng:
+    with checked(use_graph=True) as eng:
         for rep in range(3):
             got = eng.run(mix)
             assert all(_same(a, b) for a, b in zip(got, eager)), rep
@@ -136,7 +148,7 @@ def test_graph_replays_equal_eager():
 
 def test_fused_equals_per_kind():
     mix = _mix(60, 13)
-    with BatchEngine() as eng:
+    with checked() as eng:
         t = eng.tuning
         t.fused = False
         eng.tuning = t
@@ -150,7 +162,7 @@ def test_fused_equals_per_kind():
 
 def test_chunk_size_changes_rounding_only():
     mix = _mix(40, 14)
-    with BatchEngine() as eng:
+    with checked() as eng:
         r1 = eng.run(mix)
         t = eng.tuning
         t.log2_chunk = {"real": 7, "complex": 6, "int": 8}
@@ -165,9 +177,9 @@ def test_chunk_size_changes_rounding_only():
 
 def test_small_batches_and_slots():
     mix = _mix(50, 15)
-    with BatchEngine(slots=3, max_requests=4) as eng:
+    with checked(slots=3, max_requests=4) as eng:
         got = eng.run(mix)
-    with BatchEngine(slots=1) as eng:
+    with checked(slots=1) as eng:
         ref = eng.run(mix)
     assert all(_same(a, b) for a, b in zip(got, ref))
 
@@ -176,7 +188,7 @@ def test_serving_poisson_results_equal_alone():
     stream = W.serving_stream(300, rate=3000.0, seed=16, n_lo=8, n_hi=22)
     reqs = [(q.kind, q.a, q.rid) for q in stream]
     arr = np.array([q.arrival for q in stream])
-    with BatchEngine(slots=2) as eng:
+    with checked(slots=2) as eng:
         rc, res, lat, stats = eng.replay(reqs, arr)
         assert rc == 0 and stats.completed == 300
         assert stats.batches > 1  # requests arrived while others ran
@@ -187,7 +199,7 @@ def test_serving_poisson_results_equal_alone():
 
 
 def test_submit_poll_cancel_on_device():
-    with BatchEngine(slots=1, max_requests=1) as eng:
+    with checked(slots=1, max_requests=1) as eng:
         big = make("complex", 26, 1)
         small = make("real", 10, 2)
         t1 = eng.submit("complex", big)
@@ -209,7 +221,7 @@ def test_submit_poll_cancel_on_device():
 def test_autotune_keeps_results(engine):
     sample = _mix(80, 17)
     before = engine.run(sample)
-    with BatchEngine() as eng:
+    with checked() as eng:
         best, sec = eng.autotune(sample, reps=1)
         assert sec > 0 and 2 <= min(best.log2_chunk.values())
         after = eng.run(sample)
@@ -228,7 +240,8 @@ def test_rerun_after_a_failed_check_gives_the_same_results():
         assert rc == 0
         eng.inject_bad_results(5)
         rc, again, _, st = eng.replay(reqs, np.zeros(60))
-        assert rc == 0 and st.retried >= 5 and st.completed == 60
+        assert rc == 0 and st.retried == 5 and st.completed == 60 and st.late_publish == 0
+        assert eng.counters() == {"late_publish": 0, "retried": 5}
         for q, a, b in zip(reqs, first, again):
             assert _same(to_permanent(a, q[0]), to_permanent(b, q[0]))
 
@@ -239,7 +252,7 @@ def test_many_batches_every_result_checked():
     stream = W.serving_stream(3000, rate=4e4, seed=33, n_lo=8, n_hi=22)
     reqs = [(q.kind, q.a, q.rid) for q in stream]
     arr = np.array([q.arrival for q in stream])
-    with BatchEngine(slots=2, max_requests=32) as eng:
+    with checked(slots=2, max_requests=32) as eng:
         rc, res, _, st = eng.replay(reqs, arr)
     assert rc == 0 and st.completed == 3000 and st.batches >= 3000 // 32  # at most 32 per batch
     calls = {"real": api.dense_f64, "complex": api.dense_c128, "int01": lambda a: api.int_exact(a.astype(np.int64))}
@@ -250,3 +263,25 @@ def test_many_batches_every_result_checked():
             assert b.exact == p.exact
         else:
             assert abs(b.value - p.value) <= b.err_est + p.err_est
+
+
+def test_serving_poisson_full_order_range():
+    # the serving mix's whole order range, n = 8..28: every result against
+    # the one-call path (integers equal, floating point within both error
+    # estimates and the 1e-10 bar)
+    stream = W.serving_stream(400, rate=8000.0, seed=41, n_lo=8, n_hi=28)
+    reqs = [(q.kind, q.a, q.rid) for q in stream]
+    arr = np.array([q.arrival for q in stream])
+    with checked(slots=2) as eng:
+        rc, res, lat, st = eng.replay(reqs, arr)
+    assert rc == 0 and st.completed == 400 and st.batches > 1
+    assert max(a.shape[0] for _, a, _ in reqs) == 28
+    calls = {"real": api.dense_f64, "complex": api.dense_c128, "int01": lambda a: api.int_exact(a.astype(np.int64))}
+    for (kind, a, _), r in zip(reqs, res):
+        b = to_permanent(r, kind)
+        p = calls[kind](a)
+        if kind == "int01":
+            assert b.exact == p.exact
+        else:
+            assert abs(b.value - p.value) <= b.err_est + p.err_est
+            assert rel_err(b.value, p.value) <= TOL

@@ -61,3 +61,54 @@ def test_generated_source_compiles(n, seed):
 def test_codegen_rejects_non_integers():
     with pytest.raises(ValueError):
         api.codegen_source(np.full((6, 6), 0.5))
+
+
+def _real_sparse(n, seed, per_row=4):
+    rng = np.random.default_rng(seed)
+    a = W.sparse01(n, seed) * rng.standard_normal((n, n))
+    a[a == 0] = 0.0
+    return a
+
+
+def test_real_source_literals_are_exact():
+    # every hex-float literal of the generated source parses back to one of
+    # the matrix's entries (or its negation), bit for bit, and every entry
+    # appears: nothing is rounded to a decimal string
+    import re
+    a = _real_sparse(12, 5)
+    a[3, 4] = 0.1  # not dyadic: a 6-decimal literal would be exact only by luck
+    a[7, 1] = -1.0 / 3.0
+    src = api.codegen_source_real(a, register_efficient=False)  # input order: column n-1 is never flipped
+    lits = {float.fromhex(t) for t in re.findall(r"-?0x[0-9a-f]+(?:\.[0-9a-f]*)?p[+-]\d+", src)}
+    vals = {float(v) for v in a[:, :-1].ravel() if v != 0}
+    assert vals <= {abs(v) for v in lits} | {-abs(v) for v in lits}
+    flips = set(re.findall(r"x\d+ \+= (-?0x[0-9a-f.]+p[+-]\d+);", src))
+    for t in flips:
+        v = float.fromhex(t)
+        assert v in vals or -v in vals, t
+    assert "0.1" not in src.replace("0x1", "")  # no decimal spelling of the entries
+
+
+def test_real_source_levels_follow_ut_staircase():
+    # a banded pattern in UT order: column c touches rows <= c + 2, so the
+    # generator keeps more than one suffix-product level
+    n = 24
+    a = np.zeros((n, n))
+    rng = np.random.default_rng(3)
+    for i in range(n):
+        for j in range(max(0, i - 2), min(n, i + 3)):
+            a[i, j] = rng.standard_normal()
+    src = api.codegen_source_real(a)
+    m = __import__("re").search(r"(\d+) suffix-product levels", src)
+    assert m and int(m.group(1)) >= 3
+
+
+def test_real_source_compiles_for_gfx950(tmp_path):
+    import subprocess
+    a = _real_sparse(20, 9)
+    src = api.codegen_source_real(a, precision="kahan")
+    f = tmp_path / "gen.hip"
+    f.write_text("#include <hip/hip_runtime.h>\n" + src)
+    out = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O1", "-std=c++17", "--cuda-device-only",
+                          "-c", str(f), "-o", str(tmp_path / "gen.o")], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr[-2000:]

@@ -217,3 +217,55 @@ def test_golden_exact_values_reproduce():
         if "exact" in c and c["n"] <= 24:
             v, st = O.int_exact(golden_matrix(c).astype(np.int64))
             assert str(v) == c["exact"] and st == c.get("exact_status", 0), c["name"]
+
+
+@pytest.mark.parametrize("n", [5, 12, 18])
+def test_dense_ref_sample_full_range_is_the_whole_walk(n):
+    # the CPU-baseline sample over every Gray index [1, 2^(n-1)) is the
+    # reference walk's sum without the k=0 term and the final factor: with
+    # one thread both run the same additions in the same order
+    a = W.dense_normal(n, 300 + n)
+    x = [a[i, n - 1] - sum(a[i, j] for j in range(n)) / 2 for i in range(n)]  # left-to-right row sums
+    p0 = 1.0
+    for v in x:
+        p0 *= v
+    part = O.dense_ref_sample(a, 1, (1 << (n - 1)) - 1)
+    assert (4 * (n % 2) - 2) * (p0 + part) == O.dense_ref(a, 1)
+    # thread splits of the sample change the addition order only
+    assert rel_err(O.dense_ref_sample(a, 4, (1 << (n - 1)) - 1), part) <= 1e-12
+
+
+def test_dense_ref_sample_prefix_vs_direct_subset_sum():
+    # a prefix of the walk against the terms formed from scratch: k = 1..count,
+    # S = Gray(k) = k ^ (k >> 1), term (-1)^k prod_i (x_i + sum_{j in S} a_ij)
+    n, count = 11, 300
+    a = W.dense_uniform(n, 77)
+    x = a[:, n - 1] - a.sum(axis=1) / 2
+    direct = 0.0
+    for k in range(1, count + 1):
+        g = k ^ (k >> 1)
+        cols = [j for j in range(n - 1) if (g >> j) & 1]
+        direct += (-1) ** k * np.prod(x + a[:, cols].sum(axis=1))
+    assert rel_err(O.dense_ref_sample(a, 1, count), direct) <= 1e-12
+    assert rel_err(O.dense_ref_sample(a, 3, count), direct) <= 1e-12
+
+
+def test_complex_ref_sample_full_range_is_the_whole_walk():
+    n = 12
+    z = W.complex_gauss(n, 91)
+    x = z[:, n - 1] - z.sum(axis=1) / 2
+    p0 = complex(1.0, 0.0)
+    for v in x:
+        p0 = complex(p0.real * v.real - p0.imag * v.imag, p0.real * v.imag + p0.imag * v.real)
+    part = O.complex_ref_sample(z, 1, (1 << (n - 1)) - 1)
+    whole = O.complex_ref(z, 1)
+    f = 4 * (n % 2) - 2
+    assert rel_err(f * (p0 + part), whole) <= 1e-13
+    # a prefix against terms formed from scratch
+    count = 200
+    direct = 0j
+    for k in range(1, count + 1):
+        g = k ^ (k >> 1)
+        cols = [j for j in range(n - 1) if (g >> j) & 1]
+        direct += (-1) ** k * np.prod(x + z[:, cols].sum(axis=1))
+    assert rel_err(O.complex_ref_sample(z, 2, count), direct) <= 1e-12
