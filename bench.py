@@ -13,12 +13,19 @@ permanent).  The line also carries the other halves of the metric:
 uniform [0,1), 30% complex Gaussian, 20% 0/1 density 0.5, seed 2026) through
 the batched persistent kernel (autotuned, hipGraph launches, continuous
 batching) at 80% of its measured capacity, with p50/p99 latency, against the
-plain one-call-per-matrix GPU path on the same requests; and `secondary` --
-the dense n=24, sparse 0/1 n=40 (generated kernel) and Haar complex n=32
-configurations, one permanent each with inputs resident.
+plain one-call-per-matrix GPU path on the same requests, with the speed-up
+that removing every per-call cost would give (`compute_bound`) and both
+paths' latency at one offered load the plain path sustains (`equal_load`);
+`serving_small_n` -- the same for orders 8..20, where a plain call is
+mostly per-call cost; and `secondary` -- the dense n=24, sparse 0/1 n=40
+(generated kernel) and Haar complex n=32 configurations, one permanent each
+with inputs resident.
 
-roofline.achieved counts the algorithmic 2n FP64 flops per subset (n adds + n-1 multiplies + 1 accumulate) over the kernel's duration
-measured with HIP events on its own stream; peak = MI355X FP64 vector.
+roofline.achieved counts the algorithmic 2n FP64 flops per subset (n adds +
+n-1 multiplies + 1 accumulate) over the kernel's duration measured with HIP
+events on its own stream; peak = MI355X FP64 vector; ceiling = the issue
+ceiling 2n/(2(2n-1)) of peak of any Gray-code Ryser (2n-1 FP64 instructions
+per subset).
 cpu_baseline = the reference's CPU Gray-code walk (restated in oracle/, same
 chunking and arithmetic order) on a bounded sample of the same walk, timed
 on this host's cores and scaled to permanents per second.
@@ -216,12 +223,38 @@ def measured_counters(kernel: str, kernel_ms: float):
             "valu_busy_pmc": c["valu_busy"], "source": c["source"]}
 
 
-def serving(count: int, seed: int = 2026, load: float = 0.8) -> dict:
+def request_flops(kind: str, n: int) -> float:
+    """Algorithmic FP64 work of one request: 2n flops per Gray subset (real
+    and the exact FP64 walk of 0/1 matrices), 8n (complex)."""
+    return (8.0 if kind == "complex" else 2.0) * n * float(1 << (n - 1))
+
+
+def plain_replay(reqs, arrivals, calls) -> np.ndarray:
+    """One C-ABI call per request, each started at its arrival time (or as
+    soon as the previous call returns): arrival -> result latencies."""
+    lat = np.zeros(len(reqs))
+    t0 = time.perf_counter()
+    for i, ((kind, a, _), t_arr) in enumerate(zip(reqs, arrivals)):
+        while time.perf_counter() - t0 < t_arr:
+            pass
+        calls[kind](a)
+        lat[i] = time.perf_counter() - t0 - t_arr
+    return lat
+
+
+def serving(count: int, seed: int = 2026, load: float = 0.8, n_lo: int = 8, n_hi: int = 28,
+            valu_tflops: float | None = None, equal_load_count: int = 5000) -> dict:
+    """The serving stream through the batch engine (capacity, then Poisson
+    arrivals at `load` of it) against one C-ABI call per matrix.
+    `speedup_bound` = the plain path's wall time over the stream's
+    algorithmic work at `valu_tflops` (the dense n=40 kernel's measured
+    rate): what batching could reach if it removed every per-call cost and
+    ran every request at that rate."""
     from superman_amd import api
     from superman_amd import workloads as W
     from superman_amd.batch import BatchEngine, to_permanent
 
-    stream = W.serving_stream(count, rate=1.0, seed=seed)
+    stream = W.serving_stream(count, rate=1.0, seed=seed, n_lo=n_lo, n_hi=n_hi)
     reqs = [(q.kind, q.a, q.rid) for q in stream]
     unit_arrivals = np.array([q.arrival for q in stream])  # rate 1/s; scaled to the offered rate below
     with BatchEngine() as eng:
@@ -260,9 +293,36 @@ def serving(count: int, seed: int = 2026, load: float = 0.8) -> dict:
         else:
             worst = max(worst, abs(b.value - p.value) / abs(p.value))
     pl = np.sort(np.array(plain_lat))
+    # latency at an equal offered load that the plain path can sustain (80%
+    # of its own capacity): open-loop Poisson arrivals through both paths
+    m = min(equal_load_count, count)
+    rate_eq = load * plain_rate
+    arr_eq = unit_arrivals[:m] / rate_eq
+    lat_plain = np.sort(plain_replay(reqs[:m], arr_eq, calls))
+    with BatchEngine() as eng:
+        eng.tuning = tuning
+        rc, _, lat_b, st_eq = eng.replay(reqs[:m], arr_eq)
+        if rc != 0:
+            raise RuntimeError(f"batched equal-load run failed: status {rc} ({eng.last_error()})")
+        health_eq = eng.counters()
+    if health_eq["late_publish"] or health_eq["retried"]:
+        raise RuntimeError(f"batch engine health: {health_eq}")
+    equal_load = {"requests": m, "offered_rate": rate_eq,
+                  "plain_p50_ms": float(lat_plain[m // 2] * 1e3), "plain_p99_ms": float(lat_plain[int(0.99 * (m - 1))] * 1e3),
+                  "batched_p50_ms": st_eq.p50_s * 1e3, "batched_p99_ms": st_eq.p99_s * 1e3,
+                  "rule": "the first requests of the same stream, Poisson arrivals at 80% of the plain path's "
+                          "capacity, through one call per matrix (open loop) and through the batch engine"}
+    work_flops = sum(request_flops(kind, a.shape[0]) for kind, a, _ in reqs)
+    bound = None
+    if valu_tflops:
+        work_s = work_flops / (valu_tflops * 1e12)
+        bound = {"speedup_bound": plain_s / work_s, "work_s_at_valu_rate": work_s, "plain_s": plain_s,
+                 "valu_tflops": valu_tflops, "work_flops": work_flops,
+                 "rule": "plain wall time / (algorithmic FP64 flops of the stream at the dense n=40 kernel's "
+                         "measured TFLOP/s): the ratio with no per-call cost at all"}
     return {
-        "workload": f"{count} requests, Poisson arrivals, n uniform 8..28, 50% real uniform[0,1) / 30% complex "
-                    f"Gaussian / 20% 0-1 density 0.5 (exact), numpy default_rng({seed})",
+        "workload": f"{count} requests, Poisson arrivals, n uniform {n_lo}..{n_hi}, 50% real uniform[0,1) / 30% "
+                    f"complex Gaussian / 20% 0-1 density 0.5 (exact), numpy default_rng({seed})",
         "perms_per_s": st.perms_per_s, "offered_rate": rate, "load": load,
         "p50_ms": st.p50_s * 1e3, "p99_ms": st.p99_s * 1e3, "max_ms": st.max_s * 1e3,
         "batches": st.batches, "mean_batch": st.mean_batch, "device_busy": st.device_busy,
@@ -273,6 +333,8 @@ def serving(count: int, seed: int = 2026, load: float = 0.8) -> dict:
         "speedup_vs_plain": sat.perms_per_s / plain_rate,
         "max_rel_diff_vs_plain": worst, "int_mismatches_vs_plain": mismatched,
         "tuning": {"fused": tuning.fused, "waves_per_cu": tuning.waves_per_cu, "log2_chunk": tuning.log2_chunk},
+        "compute_bound": bound,
+        "equal_load": equal_load,
     }
 
 
@@ -373,11 +435,23 @@ def main(argv=None):
     other = "dd" if args.precision == "kahan" else "kahan"
     with S.DensePlan(a, other) as p2:
         r_other = p2.run()
+    # the same permanent with chunks 64 times shorter (exact rebuilds at other
+    # places, so every factor is rounded differently): a check of the error
+    # estimate that shares no products with the timed run
+    with S.DensePlan(a, args.precision) as p3:
+        L_short = max(2, info["log2_chunk"] - 6)
+        p3.set_log2_chunk(L_short)
+        r_short = p3.run()
 
     ms_per_step = dt * 1e3 / args.steps
     flops = 2.0 * n * float(1 << (n - 1))  # algorithmic 2n flops per subset
     k_ms = float(np.mean(kernel_ms))
     achieved = flops / (k_ms * 1e-3) / 1e12
+    # Issue ceiling of any Gray-code Ryser under the 2n convention: at least
+    # 2n-1 FP64 VALU instructions per subset (n adds, n-2 multiplies, one
+    # FMA into the sum); the peak counts 2 flops (an FMA) per instruction
+    # slot, so 2n algorithmic flops in 2n-1 slots reach 2n/(2(2n-1)) of it.
+    ceiling = PEAK_FP64_TFLOPS * (2.0 * n) / (2.0 * (2.0 * n - 1.0))
     out = {
         "metric": METRIC,
         "value": 1e3 / ms_per_step,
@@ -397,18 +471,28 @@ def main(argv=None):
                    "n": n, "precision": args.precision, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(info["kernel"]),
+                     "ceiling": ceiling, "frac_of_ceiling": achieved / ceiling,
+                     "ceiling_rule": "2n-1 FP64 VALU instructions per subset (n adds, n-2 multiplies, 1 FMA) at "
+                                     "the peak's issue rate: 2n/(2(2n-1)) of peak; >= 60% of peak is out of reach "
+                                     "for any Gray-code Ryser under the 2n-flop convention",
                      "kernel": info["kernel"], "kernel_ms": k_ms,
                      "flops_per_launch": flops, "flop_convention": "2n per subset (add, mul, accumulate; FMA = 2)",
                      "counters": measured_counters(info["kernel"], k_ms)},
         "result": {"value": r.value, "err_est": r.err_est, "cond": r.cond,
-                   f"{other}_value": r_other.value, f"{other}_kernel_ms": r_other.kernel_ms},
+                   f"{other}_value": r_other.value, f"{other}_kernel_ms": r_other.kernel_ms,
+                   "other_chunk_log2": L_short, "other_chunk_value": r_short.value,
+                   "other_chunk_rel_diff": abs(r_short.value - r.value) / abs(r.value),
+                   "other_chunk_within_estimates": bool(abs(r_short.value - r.value) <= r.err_est + r_short.err_est)
+                   if args.precision == "kahan" else None},
         "geometry": info,
         "device": dev,
     }
     if not args.no_secondary:
         out["secondary"] = secondary()
     if not args.no_serving:
-        out["serving"] = serving(args.serving_requests)
+        out["serving"] = serving(args.serving_requests, valu_tflops=achieved)
+        # small orders, where each plain call is mostly per-call cost
+        out["serving_small_n"] = serving(args.serving_requests, seed=2027, n_lo=8, n_hi=20, valu_tflops=achieved)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a)
     print(json.dumps(out), flush=True)

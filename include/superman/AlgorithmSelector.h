@@ -1,9 +1,16 @@
 // AlgorithmSelector.h -- CommonFiles/AlgorithmSelector.h:38-141 over the
-// drop-in GPU entry points: same function-pointer type, same `auto` rule
-// (:52-67: dense xregister_mshared when nnz/row <= 6 or the biggest
-// component is small, else register-efficient code generation) and the same
-// sparse-vs-dense routing (:97-139).  The CPU mode and the recompile mailbox
-// of the reference are not part of this library.
+// drop-in GPU entry points: same function-pointer type and the same
+// sparse-vs-dense routing (:97-139).  The `auto` rule (:52-67: dense
+// xregister_mshared when nnz/row <= 6 or the biggest component is small,
+// else register-efficient code generation) weighs the reference's own
+// GPUs; here it is retuned for MI355X from the library's cost model
+// (spm_choose_path: per-subset rates measured on MI355X for every walk and
+// generated kernel, plus the hiprtc compile of a kernel not yet cached):
+// one generated kernel for the whole matrix when that is estimated
+// cheapest, otherwise the decomposed driver, which makes the same estimate
+// per leaf (exact integer walks, row-scaled integers, generated kernels,
+// the dense FP64 walk).  The CPU mode and the recompile mailbox of the
+// reference are not part of this library.
 #ifndef SUPERMAN_AMD_ALGORITHMSELECTOR_H
 #define SUPERMAN_AMD_ALGORITHMSELECTOR_H
 
@@ -17,13 +24,8 @@ public:
 
     static Algorithm selectAlgorithm(Matrix<S>* matrix, Settings* settings, unsigned biggestComponentSize)
     {
-        if (settings->algorithm == AlgorithmEnds && settings->mode != Mode::CPU) {
-            if ((matrix->nov * matrix->sparsity / 100 <= 6) || (biggestComponentSize <= 41) ||
-                (biggestComponentSize <= (unsigned)(matrix->nov - 3)))
-                settings->algorithm = XREGISTERMSHARED;
-            else
-                settings->algorithm = REGEFFICIENTCODEGENERATION;
-        }
+        if (settings->algorithm == AlgorithmEnds && settings->mode != Mode::CPU)
+            settings->algorithm = superman_amd::auto_algorithm(matrix, biggestComponentSize);
         if (settings->mode == Mode::CPU)
             throw std::runtime_error("The CPU mode is not part of the GPU library.\n");
         const bool sparse =

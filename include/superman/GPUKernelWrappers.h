@@ -70,6 +70,25 @@ inline std::vector<double> as_double(const Matrix<S>* m)
     return a;
 }
 
+// The MI355X `auto` choice (AlgorithmSelector): REGEFFICIENTCODEGENERATION
+// when the whole matrix fits one kernel (order <= 63) and spm_choose_path
+// estimates a generated kernel cheapest; else XREGISTERMSHARED, whose
+// decomposed driver chooses per leaf.  biggestComponentSize is kept for the
+// reference's signature; the per-leaf choice makes it unnecessary.
+template <class S>
+inline Algorithm auto_algorithm(const Matrix<S>* m, unsigned biggestComponentSize)
+{
+    (void)biggestComponentSize;
+    if (m->nov >= 1 && m->nov <= 63) {
+        const std::vector<double> a = as_double(m);
+        int path = 0;
+        if (spm_choose_path(a.data(), m->nov, &path) == SPM_OK &&
+            (path == SPM_PATH_GEN_INT || path == SPM_PATH_GEN_REAL))
+            return REGEFFICIENTCODEGENERATION;
+    }
+    return XREGISTERMSHARED;
+}
+
 inline int precision_of(const Settings& s)
 {
     if (s.calculationPrecision == DD) return SPM_PREC_PLAIN;
@@ -93,10 +112,16 @@ public:
     double permanentFunction() override
     {
         const std::vector<double> a = superman_amd::as_double(this->m_Matrix);
-        superman_amd::check(spm_permanent_decomposed(a.data(), this->m_Matrix->nov, SPM_DECOMP_COMPRESS,
-                                                     superman_amd::precision_of(this->m_Settings),
-                                                     this->m_Settings.deviceID, nullptr, &last),
-                            "spm_permanent_decomposed");
+        const Settings& s = this->m_Settings;
+        if (s.scaling)  // every leaf balanced first (DecomposePerman.h:97-107, 189-193)
+            superman_amd::check(spm_permanent_decomposed_scaled(a.data(), this->m_Matrix->nov, SPM_DECOMP_COMPRESS,
+                                                                superman_amd::precision_of(s), (int)s.scalingIterationNo,
+                                                                (double)s.scaleInto, s.deviceID, nullptr, &last),
+                                "spm_permanent_decomposed_scaled");
+        else
+            superman_amd::check(spm_permanent_decomposed(a.data(), this->m_Matrix->nov, SPM_DECOMP_COMPRESS,
+                                                         superman_amd::precision_of(s), s.deviceID, nullptr, &last),
+                                "spm_permanent_decomposed");
         return last.re;
     }
     spm_result last{};
@@ -133,7 +158,27 @@ public:
         std::vector<double> b((size_t)n * n);
         spm_decomp_leaf(d, 0, nullptr, nullptr, b.data());
         spm_decomp_destroy(d);
+        std::vector<double> rs, cs;
+        if (this->m_Settings.scaling) {  // the leaf balanced first, as DecomposePerman::addQueue does
+            rs.resize(n);
+            cs.resize(n);
+            superman_amd::check(spm_sinkhorn_scale(b.data(), n, (int)this->m_Settings.scalingIterationNo,
+                                                   (double)this->m_Settings.scaleInto, rs.data(), cs.data()),
+                                "spm_sinkhorn_scale");
+        }
         for (size_t e = 0; e < b.size(); ++e) leaf->mat[e] = (S)b[e];
+        const double v = leaf_permanent(leaf, b, n);
+        for (int i = 0; i < (int)rs.size(); ++i) {  // DecomposePerman.h:97-107
+            last.re /= rs[i];
+            last.re /= cs[i];
+        }
+        return rs.empty() ? v : last.re;
+    }
+    spm_result last{};
+
+private:
+    double leaf_permanent(Matrix<S>* leaf, std::vector<double>& b, int n)
+    {
         std::vector<int64_t> ai;
         const int opt = this->m_Settings.algorithm == REGEFFICIENTCODEGENERATION ? SPM_CG_REGISTER_EFFICIENT : 0;
         if (n >= 5 && superman_amd::integral(leaf, ai)) {
@@ -150,12 +195,13 @@ public:
             superman_amd::check(st, "spm_sparse_codegen_real");
             return last.re;
         }
-        DecomposedGPU<C, S> generic(leaf, this->m_Settings);
+        Settings plain = this->m_Settings;
+        plain.scaling = false;  // a scaled leaf is not scaled again
+        DecomposedGPU<C, S> generic(leaf, plain);
         const double v = generic.computePermanent();
         last = generic.last;
         return v;
     }
-    spm_result last{};
 };
 
 template <class Job, class C, class S>

@@ -46,13 +46,18 @@ extern "C" {
 #define SPM_PREC_F32_PRODUCT  2   /* real, nonnegative entries only (else
                                      SPM_EINVAL): rows scaled by powers of two,
                                      FP64 row sums, FP32 products, compensated
-                                     FP64 accumulation; err_est bounds the FP32
-                                     product error (no reference counterpart) */
+                                     FP64 accumulation; err_est adds the FP32
+                                     product error to the FP64 estimate (no
+                                     reference counterpart) */
 
 /* Extended result.  `re` (and `im` for complex) is the value the reference
  * returns; hi/lo carry the double-double sum; `abs_sum` = 2 * sum |term|
  * (compensated modes) so that cond = abs_sum / |value|; `err_est` is an
- * a-priori bound on the absolute error; `exact_*` hold the exact integer
+ * estimate of the absolute error, about 2n roundings of u = 2^-53 per term
+ * (2n u abs_sum real, 4n u abs_sum complex): it counts each factor's
+ * rounding once, not the drift of the up to 2^L incremental updates a
+ * factor carries since its chunk's exact start, so it is no guaranteed
+ * bound (-1 in PLAIN mode, 0 for exact results); `exact_*` hold the exact integer
  * value (two's complement 128-bit) when `is_exact` is set. */
 typedef struct spm_result {
     double re, im;
@@ -256,6 +261,18 @@ int spm_decomp_destroy(spm_decomp* d);
  * (CommonFiles/DecomposePerman.h:76-117) around the GPU algorithms. */
 int spm_permanent_decomposed(const double* a, int n, int options, int precision, int device, void* stream,
                              spm_result* out);
+/* The same with the reference's `scaling=true` (Settings scaling,
+ * scalingIterationNo, scaleInto): every leaf is balanced first -- IO::scale
+ * (IO/IO.h:109-161) as DecomposePerman::addQueue applies it
+ * (CommonFiles/DecomposePerman.h:189-193) -- and its permanent divided by
+ * the row and column scales afterwards (DecomposePerman.h:97-107).  Scaled
+ * leaves are real, so the result is FP64 (is_exact = 0). */
+int spm_permanent_decomposed_scaled(const double* a, int n, int options, int precision, int scaling_iterations,
+                                    double scale_into, int device, void* stream, spm_result* out);
+/* IO::scale on its own (host only): `iterations` rounds of row then column
+ * balancing towards line sums `target`, then a is scaled in place
+ * (a_ij r_i c_j); row_scale / col_scale receive r and c. */
+int spm_sinkhorn_scale(double* a, int n, int iterations, double target, double* row_scale, double* col_scale);
 
 /* ---- batched and served permanents -----------------------------------------
  * No single counterpart in the reference, which computes one matrix per

@@ -154,3 +154,32 @@ def int_exact(a, threads: int = 8) -> tuple[int, int]:
 def int_log2_bound(a) -> float:
     a = np.ascontiguousarray(a, dtype=np.int64)
     return lib().orc_int_log2_bound(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.shape[0])
+
+
+def sinkhorn_scale(a, iterations: int = 100, target: float = 2.0):
+    """Restates IO::scale (IO/IO.h:109-161) in plain Python floats (small
+    matrices only): `iterations` rounds of row then column balancing, each
+    line scale = target / sum_k (a * r_i) * c_j where that sum is nonzero,
+    then the final in-place pass over (i, j) in row order multiplying a[i][j]
+    by r_i and a[j][i] by c_i.  Returns (scaled, r, c)."""
+    a = [[float(v) for v in row] for row in np.asarray(a, dtype=np.float64)]
+    n = len(a)
+    r, c = [1.0] * n, [1.0] * n
+    for _ in range(iterations):
+        for i in range(n):
+            s = 0.0
+            for j in range(n):
+                s += a[i][j] * r[i] * c[j]
+            if s != 0:
+                r[i] = float(target) / s
+        for j in range(n):
+            s = 0.0
+            for i in range(n):
+                s += a[i][j] * r[i] * c[j]
+            if s != 0:
+                c[j] = float(target) / s
+    for i in range(n):
+        for j in range(n):
+            a[i][j] *= r[i]
+            a[j][i] *= c[i]
+    return np.array(a), np.array(r), np.array(c)

@@ -166,6 +166,9 @@ PROTOTYPES = {
     "spm_decomp_leaf": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _D]),
     "spm_decomp_destroy": (ctypes.c_int, [_VP]),
     "spm_permanent_decomposed": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
+    "spm_permanent_decomposed_scaled": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                       ctypes.c_double, ctypes.c_int, _VP, _R]),
+    "spm_sinkhorn_scale": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D]),
     "spm_batch_config_default": (None, [ctypes.POINTER(SpmBatchConfig)]),
     "spm_batch_create": (ctypes.c_int, [ctypes.POINTER(SpmBatchConfig), ctypes.POINTER(_VP)]),
     "spm_batch_destroy": (ctypes.c_int, [_VP]),

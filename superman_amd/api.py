@@ -388,12 +388,32 @@ def decompose(a, options: int = _lib.SPM_DECOMP_DEFAULT) -> Decomposition:
 
 
 def permanent_decomposed(a, precision="kahan", options: int = _lib.SPM_DECOMP_DEFAULT, device: int = 0,
-                         stream: int | None = None) -> Permanent:
+                         stream: int | None = None, scaling: tuple[int, float] | None = None) -> Permanent:
     """perm(a) through the decomposition, each leaf on the GPU (exact for
-    integer matrices while 128 bits hold the result)."""
+    integer matrices while 128 bits hold the result).  scaling =
+    (iterations, scale_into) balances every leaf first, as the reference's
+    scaling=true does (spm_permanent_decomposed_scaled)."""
     a = _square(a, np.float64)
     r = SpmResult()
-    check(lib().spm_permanent_decomposed(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0], options,
-                                         _prec(precision), device, stream, ctypes.byref(r)),
-          "spm_permanent_decomposed")
+    ap = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    if scaling is None:
+        check(lib().spm_permanent_decomposed(ap, a.shape[0], options, _prec(precision), device, stream,
+                                             ctypes.byref(r)), "spm_permanent_decomposed")
+    else:
+        it, into = scaling
+        check(lib().spm_permanent_decomposed_scaled(ap, a.shape[0], options, _prec(precision), int(it), float(into),
+                                                    device, stream, ctypes.byref(r)),
+              "spm_permanent_decomposed_scaled")
     return _to_perm(r, False)
+
+
+def sinkhorn_scale(a, iterations: int = 100, target: float = 2.0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The reference's IO::scale on one matrix (host only): (scaled matrix,
+    row scales, column scales)."""
+    b = _square(a, np.float64).copy()
+    n = b.shape[0]
+    rs, cs = np.zeros(n), np.zeros(n)
+    D = ctypes.POINTER(ctypes.c_double)
+    check(lib().spm_sinkhorn_scale(b.ctypes.data_as(D), n, int(iterations), float(target), rs.ctypes.data_as(D),
+                                   cs.ctypes.data_as(D)), "spm_sinkhorn_scale")
+    return b, rs, cs

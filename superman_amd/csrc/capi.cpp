@@ -197,14 +197,20 @@ static void plan_free(spm_plan* p)
 // blocks as the tasks fill, at most the resident ones; very long runs
 // (~> 4 s of VALU work at 64 lanes x 256 CUs x 2.4 GHz) are split into
 // several launches that accumulate into the same per-block records.
+namespace spm {
+void dense_launch_shape(int kind, int n, int precision, uint64_t tasks, int max_grid, int* grid, int* slices)
+{
+    const uint64_t need_blocks = (tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
+    *grid = (int)std::min<uint64_t>(uint64_t(max_grid), need_blocks);
+    const double ops = precision == SPM_PREC_F32_PRODUCT ? 3.0 * n : kind == 2 ? 6.0 * n : 2.0 * n;
+    const double est_s = std::ldexp(1.0, n - 1) * ops / 3.9e13;
+    *slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)tasks);
+}
+}  // namespace spm
+
 static void dense_grid(spm_plan* p)
 {
-    const int n = p->n;
-    const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
-    p->grid = (int)std::min<uint64_t>(uint64_t(p->max_grid), need_blocks);
-    const double ops = p->precision == SPM_PREC_F32_PRODUCT ? 3.0 * n : p->kind == 2 ? 6.0 * n : 2.0 * n;
-    const double est_s = std::ldexp(1.0, n - 1) * ops / 3.9e13;
-    p->slices = (int)std::min<double>(std::max(1.0, std::ceil(est_s / 4.0)), (double)p->tasks);
+    dense_launch_shape(p->kind, p->n, p->precision, p->tasks, p->max_grid, &p->grid, &p->slices);
 }
 
 // Dense plans, real (kind 1) and complex (kind 2).  `a` is row-major n x n,

@@ -394,6 +394,38 @@ void decompose_block(Mat m, bool codegen, int above, std::vector<Mat>& leaves)
 }
 
 }  // namespace
+
+// Sinkhorn-style balancing of one leaf (the reference's IO::scale,
+// IO/IO.h:109-161, applied per leaf by DecomposePerman::addQueue,
+// CommonFiles/DecomposePerman.h:189-193): `iterations` rounds of row then
+// column passes, each setting a scale to target / (scaled line sum) where
+// that sum is nonzero, then a scaled in place.  The arithmetic follows the
+// reference's so that the scales, and the scaled leaf, are the same
+// numbers: line sums accumulate (a_ij * r_i) * c_j left to right, and the
+// final pass visits (i, j) in row order, multiplying a_ij by r_i and a_ji
+// by c_i -- so an entry below the diagonal gets its column scale first.
+void sinkhorn_scale(double* a, int n, int iterations, double target, double* rs, double* cs)
+{
+    for (int i = 0; i < n; ++i) rs[i] = cs[i] = 1.0;
+    for (int it = 0; it < iterations; ++it) {
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += a[(size_t)i * n + j] * rs[i] * cs[j];
+            if (s != 0.0) rs[i] = target / s;
+        }
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int i = 0; i < n; ++i) s += a[(size_t)i * n + j] * rs[i] * cs[j];
+            if (s != 0.0) cs[j] = target / s;
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            a[(size_t)i * n + j] *= rs[i];
+            a[(size_t)j * n + i] *= cs[i];
+        }
+}
+
 }  // namespace spm
 
 using namespace spm;
@@ -496,8 +528,22 @@ extern "C" int spm_decomp_destroy(spm_decomp* d)
 
 // ---- the permanent through the decomposition ----------------------------------
 
-extern "C" int spm_permanent_decomposed(const double* a, int n, int options, int precision, int device, void* stream,
-                                        spm_result* out)
+extern "C" int spm_sinkhorn_scale(double* a, int n, int iterations, double target, double* row_scale,
+                                  double* col_scale)
+{
+    if (!a || !row_scale || !col_scale || n < 1 || iterations < 0 || !(target > 0.0) || !std::isfinite(target))
+        return SPM_EINVAL;
+    for (size_t e = 0; e < (size_t)n * n; ++e)
+        if (!std::isfinite(a[e])) return SPM_EINVAL;
+    sinkhorn_scale(a, n, iterations, target, row_scale, col_scale);
+    return SPM_OK;
+}
+
+// The decomposed permanent; scaling_iterations > 0 balances every leaf
+// first and divides its permanent by the scales afterwards (the reference's
+// DecomposePerman with Settings.scaling, DecomposePerman.h:97-107).
+static int permanent_decomposed(const double* a, int n, int options, int precision, int scaling_iterations,
+                                double scale_into, int device, void* stream, spm_result* out)
 {
     if (!out) return SPM_EINVAL;
     std::memset(out, 0, sizeof *out);
@@ -533,11 +579,28 @@ extern "C" int spm_permanent_decomposed(const double* a, int n, int options, int
         // each leaf by the automatic path choice (exact integer walks or
         // generated kernels, row-scaled integers, generated FP64 kernels,
         // the dense FP64 walk)
+        const double* leaf = m.a.data();
+        std::vector<double> scaled, rs, cs;
+        if (scaling_iterations > 0) {
+            scaled = m.a;
+            rs.resize(m.n);
+            cs.resize(m.n);
+            sinkhorn_scale(scaled.data(), m.n, scaling_iterations, scale_into, rs.data(), cs.data());
+            leaf = scaled.data();
+        }
         spm_result r;
-        st = spm_permanent_auto(m.a.data(), m.n, precision, device, stream, &r, nullptr);
+        st = spm_permanent_auto(leaf, m.n, precision, device, stream, &r, nullptr);
         if (st != SPM_OK) {
             out->status = st;
             return st;
+        }
+        if (scaling_iterations > 0) {  // perm(leaf) = perm(scaled) / prod r_i c_i, divided in the reference's order
+            for (int i = 0; i < m.n; ++i) {
+                r.re /= rs[i];
+                r.re /= cs[i];
+                if (r.err_est > 0) r.err_est /= std::fabs(rs[i] * cs[i]);
+            }
+            r.is_exact = 0;
         }
         if (!r.is_exact) exact = false;
         const int c = d->leaf_comp[k];
@@ -577,4 +640,24 @@ extern "C" int spm_permanent_decomposed(const double* a, int n, int options, int
     for (const Mat& m : d->leaves) out->subsets += uint64_t(1) << (m.n - 1);
     out->seconds = now_seconds() - t0;
     return SPM_OK;
+}
+
+extern "C" int spm_permanent_decomposed(const double* a, int n, int options, int precision, int device, void* stream,
+                                        spm_result* out)
+{
+    return permanent_decomposed(a, n, options, precision, 0, 1.0, device, stream, out);
+}
+
+extern "C" int spm_permanent_decomposed_scaled(const double* a, int n, int options, int precision,
+                                               int scaling_iterations, double scale_into, int device, void* stream,
+                                               spm_result* out)
+{
+    if (scaling_iterations < 0 || !(scale_into > 0.0) || !std::isfinite(scale_into)) {
+        if (out) {
+            std::memset(out, 0, sizeof *out);
+            out->status = SPM_EINVAL;
+        }
+        return SPM_EINVAL;
+    }
+    return permanent_decomposed(a, n, options, precision, scaling_iterations, scale_into, device, stream, out);
 }

@@ -284,7 +284,7 @@ int f32prod_scale(const double* a, int n, double* scaled)
     return total;
 }
 
-// Error bound and scale of an FP32-product result.  A term whose FP32
+// Error estimate and scale of an FP32-product result.  A term whose FP32
 // product falls below 2^-126 loses relative accuracy: its absolute error is
 // below 2^-149 |x_last| < 2^(c+1-149), over 2^(n-1) terms (times the final
 // factor 2).

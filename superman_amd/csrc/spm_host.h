@@ -25,17 +25,25 @@ double now_seconds();
 enum class GrayWalk { kOther, kReal, kComplex };
 void gray_geometry(int n, int unroll_bits, int64_t resident_waves, int* L, uint64_t* chunks, uint64_t* tasks,
                    GrayWalk walk = GrayWalk::kOther);
+// Grid and launch slices of a dense walk over `tasks` wave tasks (kind 1
+// real, 2 complex, 5 generated real): as many blocks as the tasks fill, at
+// most max_grid (the resident ones); runs longer than ~4 s of VALU work are
+// split into launches that accumulate into the same per-block records.
+void dense_launch_shape(int kind, int n, int precision, uint64_t tasks, int max_grid, int* grid, int* slices);
 
-// A-priori absolute error estimate for a compensated FP64 real run:
-// each term carries ~(n-1) product roundings plus the rounding of its n
-// incrementally-updated factors; abs_sum = 2 sum |term|.
+// Absolute error estimate (not a guaranteed bound) for a compensated FP64
+// real run: each term carries ~(n-1) product roundings plus one rounding
+// per factor; abs_sum = 2 sum |term|.  The drift of a factor over the up to
+// 2^L incremental updates since its chunk's exact start is not counted: a
+// guaranteed bound is n (2^L + 2n + 1) u prod_i R_i (R_i = sum_j |a_ij|),
+// far above the errors seen.
 inline double dense_err_est(int n, double abs_sum)
 {
     const double u = 1.1102230246251565e-16;  // 2^-53
     return 2.0 * n * u * abs_sum;
 }
 
-// The same for complex runs: abs_sum = 2 sum |p|_1 |x_last|_1 bounds the
+// The same estimate for complex runs: abs_sum = 2 sum |p|_1 |x_last|_1 bounds the
 // terms' moduli; each term carries ~2(n-1) roundings from its complex
 // products plus its factors' incremental updates.
 inline double complex_err_est(int n, double abs_sum)

@@ -85,3 +85,36 @@ def test_reference_example_matrices(name):
         assert abs(v - truth) <= r.err_est
         # compensated sums: closer to the truth than the reference's FP64 value
         assert abs(v - truth) < abs(c["value"] - truth)
+
+
+def _scaling_cases():
+    with open(os.path.join(GOLDEN_DIR, "scaling_golden.json")) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("case", _scaling_cases(), ids=lambda c: c["name"])
+@pytest.mark.parametrize("algorithm", ["auto", "register_efficient_code_generation"])
+def test_scaling_matches_reference(case, algorithm):
+    # scaling=true: every leaf balanced (IO::scale) before its permanent and
+    # the scales divided out after it; the reference binary's printed value
+    # and the unscaled truth agree with ours within 1e-10 relative
+    a = getattr(W, case["generator"])(case["n"], case["seed"])
+    rc, out, err = cli.run(a, "scaling=true", f"scaling_iteration_no={case['scaling_iteration_no']}",
+                           f"scale_into={case['scale_into']}", f"algorithm={algorithm}")
+    assert rc == 0, err
+    v = cli.value(out)
+    if case["generator"] == "sparse01":  # an integer permanent: the reference's unscaled value is exact
+        truth = case["reference_plain"]
+    else:
+        hi, lo = O.dense_dd(a)
+        truth = hi + lo
+    assert rel_err(v, truth) <= 1e-10, (v, truth)
+    assert rel_err(v, case["reference_scaled"]) <= 1e-10, (v, case["reference_scaled"])
+
+
+def test_scaled_decomposed_api_vs_unscaled():
+    a = W.sparse01(36, 36) * (1.0 + W.dense_uniform(36, 37))
+    r0 = S.permanent_decomposed(a)
+    r1 = S.permanent_decomposed(a, scaling=(100, 2.0))
+    assert r1.exact is None
+    assert abs(r1.value - r0.value) <= 2 * (r0.err_est + r1.err_est) + 1e-10 * abs(r0.value)

@@ -15,9 +15,14 @@ def test_auto_selects_dense_for_small_components():
     assert rc == 0 and out[0] == "algorithm %d sparse 0" % XREGISTERMSHARED
 
 
-def test_auto_selects_codegen_for_large_dense_component():
-    # nnz/row > 6 and biggest component (= n) > 41 and > n - 3
+def test_auto_follows_the_mi355x_cost_model():
+    # the reference's rule would pick code generation here (nnz/row > 6,
+    # biggest component 44 > 41 and > n - 3); on MI355X the dense walk is
+    # estimated cheaper than compiling and running a 1936-literal kernel
     rc, out = dropin.run("sel", np.ones((44, 44)))
+    assert rc == 0 and out[0] == "algorithm %d sparse 0" % XREGISTERMSHARED
+    # an order-48 sparse 0/1 matrix: the generated kernel pays for its compile
+    rc, out = dropin.run("sel", W.sparse01(48, 48))
     assert rc == 0 and out[0] == "algorithm %d sparse 1" % REGEFFCG
 
 

@@ -1,5 +1,5 @@
 """FP32-product / FP64-accumulate variant for nonnegative matrices
-(SPM_PREC_F32_PRODUCT): every result within its stated error bound of the
+(SPM_PREC_F32_PRODUCT): every result within its stated error estimate of the
 double-double oracle, exact power-of-two row-scaling behaviour, and the
 measured error next to the bound."""
 import math
@@ -25,7 +25,7 @@ def test_within_bound_of_oracle(n):
     hi, lo = O.dense_dd(a)
     r = f32(a)
     err = abs(r.value - (hi + lo))
-    print(f"n={n} rel err {err / abs(hi):.2e} bound {r.err_est / abs(hi):.2e}")
+    print(f"n={n} rel err {err / abs(hi):.2e} estimate {r.err_est / abs(hi):.2e}")
     assert err <= r.err_est
     assert r.err_est / abs(hi) < 1e-2
 
