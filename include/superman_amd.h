@@ -92,6 +92,18 @@ int spm_device_info(int device, int* compute_units, int* clock_khz, char* name, 
  * synchronous with respect to it. */
 int spm_dense_f64(const double* a, int n, int precision, int device, void* stream, spm_result* out);
 
+/* spm_dense_f64 without any allocation per call: no hipMalloc / hipFree /
+ * hipHostMalloc.  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Same arguments, kernels, geometry and
+ * bit-identical result as spm_dense_f64.  Device memory is one fixed array
+ * per device (191 KiB, held until the process ends: the +A/-A planes,
+ * x(empty set) and up to 4096 per-block records), set up at the first call
+ * on each device.  Limits: 1 <= n <= 63 (else SPM_EINVAL); a launch needing
+ * more than 4096 blocks returns SPM_ENOMEM.  Calls that overlap on one
+ * device share that array and race: serialize them. */
+int spm_dense_f64_noalloc(const double* a, int n, int precision, int device, void* stream, spm_result* out);
+
 /* ---- dense complex128 ----------------------------------------------------
  * Replaces gpuComputeComplex (Algorithms/Complex/gpuComputeComplex.cu:199-418,
  * kernels :39-197), reached through complexPerman (Algorithms/AllAlgorithms.h:

@@ -126,6 +126,7 @@ PROTOTYPES = {
     "spm_device_info": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.c_char_p, ctypes.c_int]),
     "spm_dense_f64": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
+    "spm_dense_f64_noalloc": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_dense_c128": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_int_exact": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, _VP, _R]),
     "spm_plan_int": (ctypes.c_int, [_I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),

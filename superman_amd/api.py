@@ -80,6 +80,18 @@ def dense_f64(a, precision="kahan", device: int = 0, stream: int | None = None) 
     return _to_perm(r, False)
 
 
+def dense_f64_noalloc(a, precision="kahan", device: int = 0, stream: int | None = None) -> Permanent:
+    """dense_f64 through spm_dense_f64_noalloc: the same result with no
+    device allocation per call (one fixed per-device array; calls that
+    overlap on one device race)."""
+    a = _square(a, np.float64)
+    r = SpmResult()
+    st = lib().spm_dense_f64_noalloc(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0],
+                                     _prec(precision), device, stream, ctypes.byref(r))
+    check(st, "spm_dense_f64_noalloc")
+    return _to_perm(r, False)
+
+
 def dense_c128(a, precision="kahan", device: int = 0, stream: int | None = None) -> Permanent:
     a = _square(a, np.complex128)
     r = SpmResult()

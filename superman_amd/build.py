@@ -34,6 +34,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 # code compiled by the same driver (HIP runtime API).
 SOURCES = [
     ("dense_f64.hip", []),
+    ("dense_noalloc.hip", []),
     ("dense_c128.hip", []),
     ("int_dense.hip", []),
     ("int_small.hip", []),
