@@ -862,6 +862,7 @@ struct Shape {
     int n = 0;
     size_t nnz = 0;
     bool integral = false;  // every entry an integer of magnitude <= 2^30
+    IntWalkCosts walks;     // integral matrices: the exact walks' per-subset costs
 };
 
 Shape shape_of(const double* a, int n)
@@ -874,24 +875,46 @@ Shape shape_of(const double* a, int n)
         s.nnz += v != 0.0 ? 1 : 0;
         if (!(std::fabs(v) <= 1073741824.0) || v != std::floor(v)) s.integral = false;
     }
+    if (s.integral) {
+        std::vector<int64_t> q((size_t)n * n);
+        for (size_t e = 0; e < q.size(); ++e) q[e] = (int64_t)a[e];
+        s.walks = int_walk_costs(q.data(), n);
+    }
     return s;
 }
 
 double subsets(int n) { return std::ldexp(1.0, n - 1); }
 double per_row(const Shape& s) { return double(s.nnz) / s.n; }
 
-// Seconds, from the kernels' measured rates on MI355X (fixed launch costs
-// of a few tens of microseconds included).
+// Seconds, from the kernels' measured rates on MI355X at orders 32-40
+// (fixed launch costs of a few tens of microseconds included; below order
+// ~30 every path is dominated by the ~0.15 ms a call costs on the host and
+// the dense or exact walks win by not compiling).  Generated kernels: per
+// subset, the flipped column's entries and the product (the real kernels'
+// cached suffix products stop helping as rows fill: order 36, 3 entries per
+// row 20 ms, dense 295 ms against 71 ms for the dense walk), and for
+// integer kernels the 128-bit product on the fraction q of terms with no
+// zero row (order 36, 3 entries per row, odd row sums: 291 ms; 4 per row:
+// 9 ms); compiles from the first-call minus cached times.
 double est_dense(const Shape& s) { return 3e-5 + subsets(s.n) * 2.0 * s.n / 3.5e13; }
 double est_int_walk(const Shape& s)
 {
-    if (s.n >= 26 && 4 * s.nnz <= (size_t)s.n * s.n)  // compressed-column walk
-        return 3e-5 + subsets(s.n) * 1.03e-12 * per_row(s) / 4.0;
-    return 3e-5 + subsets(s.n) * s.n * 6.4e-14;     // FP64 walk
+    const IntWalkCosts& c = s.walks;
+    const double walk = s.n >= 26 ? std::min(c.sparse, std::min(c.fp64, c.int32)) : std::min(c.fp64, c.int32);
+    return 3e-5 + subsets(s.n) * walk;
 }
-double est_gen_int(const Shape& s) { return 3e-5 + subsets(s.n) * 2.6e-13 * per_row(s) / 4.0; }
-double est_gen_real(const Shape& s) { return 3e-5 + subsets(s.n) * (per_row(s) + 14.0) * 3.8e-14; }
+double est_gen_int(const Shape& s)
+{
+    const double pr = per_row(s);
+    return 3e-5 + subsets(s.n) * (2.0e-14 * pr * pr + 2.3e-13 * s.n * s.walks.q);
+}
+double est_gen_real(const Shape& s)
+{
+    const double pr = per_row(s);
+    return 3e-5 + subsets(s.n) * ((pr + 14.0) * 3.8e-14 + 5e-15 * pr * pr);
+}
 double est_compile(const Shape& s) { return 0.5 + 1.6e-4 * s.n * double(s.nnz); }
+double est_compile_real(const Shape& s) { return 0.15 + 1.2e-3 * double(s.nnz); }
 
 }  // namespace
 
@@ -910,7 +933,8 @@ static int choose_path(const double* a, int n, bool gen_cached, int* path)
         *path = SPM_PATH_SCALED_INT;
         return SPM_OK;
     }
-    *path = can_gen && est_gen_real(s) + compile < est_dense(s) ? SPM_PATH_GEN_REAL : SPM_PATH_DENSE;
+    const double compile_real = gen_cached ? 0.0 : est_compile_real(s);
+    *path = can_gen && est_gen_real(s) + compile_real < est_dense(s) ? SPM_PATH_GEN_REAL : SPM_PATH_DENSE;
     return SPM_OK;
 }
 
@@ -930,7 +954,7 @@ extern "C" int spm_path_estimate(const double* a, int n, int path, double* secon
     case SPM_PATH_INT: *seconds = est_int_walk(s); break;
     case SPM_PATH_GEN_INT: *seconds = est_gen_int(s) + est_compile(s); break;
     case SPM_PATH_SCALED_INT: *seconds = est_int_walk(s); break;
-    case SPM_PATH_GEN_REAL: *seconds = est_gen_real(s) + est_compile(s); break;
+    case SPM_PATH_GEN_REAL: *seconds = est_gen_real(s) + est_compile_real(s); break;
     case SPM_PATH_DENSE: *seconds = est_dense(s); break;
     default: return SPM_EINVAL;
     }
@@ -985,8 +1009,8 @@ extern "C" int spm_permanent_auto(const double* a, int n, int precision, int dev
         }
         if (st != SPM_ERANGE) return out->status = st;
         Shape sp = shape_of(a, n);
-        path = n >= kCodegenMinN && est_gen_real(sp) + est_compile(sp) < est_dense(sp) ? SPM_PATH_GEN_REAL
-                                                                                      : SPM_PATH_DENSE;
+        path = n >= kCodegenMinN && est_gen_real(sp) + est_compile_real(sp) < est_dense(sp) ? SPM_PATH_GEN_REAL
+                                                                                           : SPM_PATH_DENSE;
     }
     if (path == SPM_PATH_GEN_REAL)
         st = spm_sparse_codegen_real(a, n, SPM_CG_REGISTER_EFFICIENT, precision, device, stream, out);

@@ -135,20 +135,63 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
         }
 }
 
+// Fraction of Gray subsets whose term has no zero row, the terms the
+// zero-skipping walks must multiply out.  y_i(S) = 2 a_{i,n-1} - r_i +
+// 2 sum_{j in S} a_ij has the parity of the row sum r_i, so a row with an odd
+// sum is never zero; a row with an even sum and k entries in the walked
+// columns 0..n-2 is zero for about C(k, k/2) / 2^k of the subsets (exact for
+// 0/1 rows, an upper estimate for larger entries), independently of the
+// other rows to a first approximation.
+double int_nonzero_term_fraction(const int64_t* a, int n)
+{
+    double q = 1.0;
+    for (int i = 0; i < n; ++i) {
+        int64_t r = 0;
+        int k = 0;
+        for (int j = 0; j < n; ++j) {
+            r += a[(size_t)i * n + j];
+            k += (j < n - 1 && a[(size_t)i * n + j] != 0) ? 1 : 0;
+        }
+        if ((r & 1) != 0 || k == 0) continue;
+        double p = 1.0;  // C(k, floor(k/2)) / 2^k
+        for (int t = 1; t <= k; ++t) p *= (t <= k / 2 ? double(k - k / 2 + t) / t : 1.0) * 0.5;
+        q *= 1.0 - std::min(1.0, p);
+    }
+    return q;
+}
+
+// Seconds per Gray subset of the exact integer walks on MI355X, fitted to
+// measured times at orders 32-40 (0/1 matrices with 3-8 entries per row,
+// config 3, dense 0/1): the compressed-column walk pays per entry of the
+// flipped column and, on the fraction q of terms with no zero row, for the
+// 128-bit product; the FP64 walk (every absolute row sum below 2^b) and the
+// int32 walk pay per row.  With odd row sums (q = 1) the sparse walk loses
+// its advantage: sparse 0/1 order 36 with 3 entries per row takes 216 ms
+// there against 33 ms with 4.
+IntWalkCosts int_walk_costs(const int64_t* a, int n)
+{
+    IntWalkCosts c;
+    size_t nnz = 0;
+    for (size_t e = 0; e < (size_t)n * n; ++e) nnz += a[e] != 0 ? 1 : 0;
+    c.per_row = double(nnz) / n;
+    c.q = int_nonzero_term_fraction(a, n);
+    c.sparse = 2.3e-13 * c.per_row + 1.6e-13 * n * c.q;
+    c.fp64 = int_small_ok(a, n) ? 6.4e-14 * n : 1e300;
+    c.int32 = 8.3e-14 * n;
+    return c;
+}
+
 // Where the sparse walk (columns in ascending_col_sort order) is the faster
-// one, measured on MI355X: matrices with at most a quarter of their entries
-// nonzero, from order 26 (below, a permanent is too small a job for its
-// per-chunk start).  Against the FP64 walk on 0/1 matrices: equal at order
-// 24 (4 entries per row), 1.5x at order 28, 1.2x on average on the config-3
-// leaves of order 30 and 2.5x on the whole order-40 matrix; with larger
-// entries it replaces the int32 walk (2.3x faster on order-30 leaves of
-// the curtis54 example).
+// one: from order 26 (below, a permanent is too small a job for its
+// per-chunk start) when the cost model above puts it below the dense exact
+// walk that applies (measured on 0/1 matrices with 4 entries per row: equal
+// at order 24, 1.5x at order 28, 2.5x on the config-3 order-40 matrix; with
+// larger entries 2.3x over the int32 walk on order-30 curtis54 leaves).
 bool int_sparse_preferred(const int64_t* a, int n)
 {
     if (n < 26) return false;
-    size_t nnz = 0;
-    for (size_t e = 0; e < (size_t)n * n; ++e) nnz += a[e] != 0 ? 1 : 0;
-    return 4 * nnz <= (size_t)n * n;
+    const IntWalkCosts c = int_walk_costs(a, n);
+    return c.sparse < std::min(c.fp64, c.int32);
 }
 
 // Rows that are scalar multiples of integer vectors: perm(A) = prod_i s_i *

@@ -67,6 +67,14 @@ void int_planes(const int64_t* a, int n, int* planes);
 void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob);
 std::vector<int64_t> ascending_col_sort(const int64_t* a, int n);
 bool int_sparse_preferred(const int64_t* a, int n);
+double int_nonzero_term_fraction(const int64_t* a, int n);
+// Seconds per Gray subset of the exact integer walks (prep.cpp), and the
+// inputs of the estimate: entries per row and the fraction q of terms with
+// no zero row.
+struct IntWalkCosts {
+    double sparse = 0, fp64 = 0, int32 = 0, per_row = 0, q = 1;
+};
+IntWalkCosts int_walk_costs(const int64_t* a, int n);
 bool int_small_ok(const int64_t* a, int n);
 bool row_scaled_integer(const double* a, int n, std::vector<int64_t>& q, double* scale_hi, double* scale_lo);
 void scaled_int_result(double scale_hi, double scale_lo, const spm_result& t, spm_result* r);
