@@ -48,7 +48,9 @@ extern "C" {
                                      FP64 row sums, FP32 products, compensated
                                      FP64 accumulation; err_est adds the FP32
                                      product error to the FP64 estimate (no
-                                     reference counterpart) */
+                                     reference counterpart).  Non-production on
+                                     MI355X: slower and less accurate than
+                                     SPM_PREC_COMPENSATED there. */
 
 /* Extended result.  `re` (and `im` for complex) is the value the reference
  * returns; hi/lo carry the double-double sum; `abs_sum` = 2 * sum |term|

@@ -59,3 +59,19 @@ def test_order_42_matches_sparse_walk():
     with api.DensePlan(a, "exact", int_walk="sparse") as q:
         s = q.run().exact
     assert g == s and g > 0
+
+
+def test_order_44_triangular_closed_form():
+    # a generated kernel past order 42 against a closed form: an upper-
+    # triangular 0/1 pattern (ones on the diagonal, 3 more entries per row
+    # above it) has perm = 1, so the 128-bit sum of 2^43 terms must cancel
+    # to exactly 2^43 (about 4 s with the compile)
+    n = 44
+    rng = np.random.default_rng(n)
+    a = np.zeros((n, n), dtype=np.int64)
+    for i in range(n):
+        a[i, i] = 1
+        above = np.arange(i + 1, n)
+        if above.size:
+            a[i, rng.choice(above, size=min(3, above.size), replace=False)] = 1
+    assert api.sparse_codegen(a).exact == 1
