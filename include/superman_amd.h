@@ -116,11 +116,12 @@ int spm_dense_c128(const double* a, int n, int precision, int device, void* stre
 
 /* ---- exact integer ---------------------------------------------------------
  * Integer matrices (|a_ij| <= 2^30, row sums of |a_ij| < 2^30) computed
- * exactly: y = 2x in int32, T = sum (-1)^k prod y (mod 2^128), perm =
- * (-1)^(n-1) T / 2^(n-1), returned in exact_lo/exact_hi (is_exact = 1) and
- * rounded into `re`.  Returns SPM_ERANGE unless 2^(n-1) times a bound on
- * |perm| (Bregman-Minc for 0/1 matrices, the product of absolute row sums
- * otherwise) stays below 2^126.
+ * exactly: y = 2x in int32 -- x itself on rows with an even sum, whose y
+ * is always even -- T = sum (-1)^k prod y (mod 2^128), perm =
+ * (-1)^(n-1) T / 2^(n-1-e) with e such rows, returned in exact_lo/exact_hi
+ * (is_exact = 1) and rounded into `re`.  Returns SPM_ERANGE unless
+ * 2^(n-1-e) times a bound on |perm| (Bregman-Minc for 0/1 matrices, the
+ * product of absolute row sums otherwise) stays below 2^126.
  * The reference has no integer path: its sparse drivers gpuSPSingleGPU
  * (Algorithms/GPU/GPUKernelWrappers.cu:65-113) -> spSingleGPU
  * (Algorithms/GPU/Sparse/spSingleGPU.cuh:44-234) and the generated-kernel

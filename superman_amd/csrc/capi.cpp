@@ -24,6 +24,7 @@
 #define SPM_CHUNK_AMORTIZE 1024  // chunk work >= this x its x-rebuild work
 #endif
 
+
 namespace spm {
 
 // ---------------------------------------------------------------------------
@@ -99,6 +100,15 @@ double now_seconds()
 // (tasks >= resident waves), large enough that rebuilding x at a chunk start
 // (~n (n-L) FMAs plus the task bookkeeping) is <~0.1% of the chunk's
 // 2^L (2n-1) ops (measured at n=40: 2^10-step chunks cost 3% more time).
+// Longer chunks still gain a little on the dense real walk (measured
+// against 2^14 steps: order 40 with 2^17 1.4% faster, 35 and 36 with 2^15
+// and 2^16 2.1% and 1.8%, 37-38 flat) but each factor then drifts over 8x
+// more incremental updates: the order-41 transposed pair moved from within
+// 1e-10 to 2.0e-10 apart, so the chunk stays.  Shorter chunks to fill the
+// last round of tasks when the resident waves are not a power of two (3 or
+// 5 per SIMD) measured slower at every order tried (real 30-36, complex
+// 26-34): a partial round runs at lower occupancy and costs less than its
+// share.
 // Where the wave count sets L (small orders), a chunk start -- n columns
 // through dependent scalar loads -- still outweighs a few more idle wave
 // slots, so the dense walks take longer chunks there (`walk`): real +2
@@ -159,6 +169,7 @@ struct spm_plan {
     double* d_M = nullptr;   // dense kinds; the integer kind stores int32 here
     double* d_x0 = nullptr;
     double log2_bound = 0;   // integer kind: log2 of the bound on |T|
+    int halved = 0;          // integer kinds: rows carried halved (int_row_weights)
     int log2_scale = 0;      // FP32-product plans: perm = 2^log2_scale perm(scaled)
     double* d_partial = nullptr;
     double* d_out = nullptr;
@@ -290,12 +301,14 @@ static int make_dense_plan(int kind, const double* a, int n, int precision, int 
     return SPM_OK;
 }
 
-// Integer checks and y(empty set) (prep.cpp), into a vector.
-static int int_prepare(const int64_t* a, int n, std::vector<int>& y0, double* log2_bound)
+// Integer checks and y(empty set) (prep.cpp), into a vector; w = row
+// weights (nullptr: no halved rows).
+static int int_prepare(const int64_t* a, int n, std::vector<int>& y0, double* log2_bound,
+                       const unsigned char* w = nullptr)
 {
     if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
     y0.assign(n, 0);
-    return spm::int_prepare(a, n, y0.data(), log2_bound);
+    return spm::int_prepare(a, n, y0.data(), log2_bound, w);
 }
 
 // Exact integer plans (kind 3): y = 2x in int32, T mod 2^128.
@@ -308,9 +321,14 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
     if (walk != SPM_INT_WALK_AUTO && walk != SPM_INT_WALK_INT32 && walk != SPM_INT_WALK_FP64 &&
         walk != SPM_INT_WALK_SPARSE)
         return SPM_EINVAL;
+    if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
+    // rows with even sums carried halved (int_row_weights); the column sort
+    // below keeps every row sum
+    std::vector<unsigned char> w(n);
+    const int halved = int_row_weights(a, n, w.data());
     std::vector<int> y0;
     double lb = 0;
-    int rc = int_prepare(a, n, y0, &lb);
+    int rc = int_prepare(a, n, y0, &lb, w.data());
     if (rc != SPM_OK) return rc;
     const bool small = int_small_ok(a, n);
     if (walk == SPM_INT_WALK_FP64 && !small) return SPM_ERANGE;
@@ -320,7 +338,7 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
     if (sp) {  // sparsest columns first: flipped most often (ascending_col_sort)
         sorted = ascending_col_sort(a, n);
         a = sorted.data();
-        rc = int_prepare(a, n, y0, &lb);
+        rc = int_prepare(a, n, y0, &lb, w.data());
         if (rc != SPM_OK) return rc;
     }
     int cus = 0;
@@ -335,6 +353,7 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
     p->precision = -1;
     p->device = device;
     p->log2_bound = lb;
+    p->halved = halved;
     p->kern = sp ? int_sparse_kernel_ptr(n) : fp ? int_small_kernel_ptr(n) : int_dense_kernel_ptr(n);
     p->rec = 2;
     p->comps = 0;
@@ -361,7 +380,7 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
     const size_t nn = (size_t)n * n;
     if (sp) {
         std::vector<int> blob;
-        int_sparse_blob(a, n, blob);
+        int_sparse_blob(a, n, blob, w.data());
         SPM_TRY(hipMalloc(&p->d_M, sizeof(int) * blob.size()));
         std::vector<int> ys(y0.begin(), y0.end());
         ys.push_back(small ? 1 : 0);  // products in FP64 groups (int_sparse_kernel)
@@ -370,14 +389,14 @@ static int make_int_plan(const int64_t* a, int n, int walk, int device, spm_plan
         SPM_TRY(hipMemcpy(p->d_x0, ys.data(), sizeof(int) * ys.size(), hipMemcpyHostToDevice));
     } else if (fp) {
         std::vector<double> colmaj(2 * nn), yd(y0.begin(), y0.end());
-        int_small_planes(a, n, colmaj.data());
+        int_small_planes(a, n, colmaj.data(), w.data());
         SPM_TRY(hipMalloc(&p->d_M, sizeof(double) * 2 * nn));
         SPM_TRY(hipMalloc(&p->d_x0, sizeof(double) * n));
         SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice));
         SPM_TRY(hipMemcpy(p->d_x0, yd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     } else {
         std::vector<int> colmaj(2 * nn);
-        int_planes(a, n, colmaj.data());
+        int_planes(a, n, colmaj.data(), w.data());
         SPM_TRY(hipMalloc(&p->d_M, sizeof(int) * 2 * nn));
         SPM_TRY(hipMalloc(&p->d_x0, sizeof(int) * n));
         SPM_TRY(hipMemcpy(p->d_M, colmaj.data(), sizeof(int) * 2 * nn, hipMemcpyHostToDevice));
@@ -469,9 +488,11 @@ extern "C" int spm_plan_codegen(const int64_t* a, int n, int options, int device
     *out = nullptr;
     if (!a || n < kCodegenMinN || n > kMaxN) return SPM_EINVAL;
     const std::vector<int64_t> b = codegen_matrix(a, n, options);
+    std::vector<unsigned char> w(n);
+    const int halved = int_row_weights(b.data(), n, w.data());  // the emitted kernel halves the same rows
     std::vector<int> y0;
     double lb = 0;
-    int rc = int_prepare(b.data(), n, y0, &lb);
+    int rc = int_prepare(b.data(), n, y0, &lb, w.data());
     if (rc != SPM_OK) return rc;
     int cus = 0;
     rc = device_cus(device, &cus);
@@ -485,6 +506,7 @@ extern "C" int spm_plan_codegen(const int64_t* a, int n, int options, int device
     p->precision = -1;
     p->device = device;
     p->log2_bound = lb;
+    p->halved = halved;
     p->rec = 2;
     p->hash = pattern_hash(b.data(), n, options);
     p->kname = codegen_name(p->hash);
@@ -714,7 +736,7 @@ static int int_run(spm_plan* p, hipStream_t st, spm_result* r)
     std::memset(r, 0, sizeof *r);
     unsigned long long w[2];
     std::memcpy(w, p->h_out, 16);
-    const int rc = int_fill_result(p->n, w[0], w[1], r);
+    const int rc = int_fill_result(p->n, w[0], w[1], r, p->halved);
     if (rc != SPM_OK) return rc;
     r->status = SPM_OK;
     r->kernel_ms = ms;

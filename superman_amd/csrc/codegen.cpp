@@ -134,12 +134,13 @@ std::string lit(int64_t v)
     return std::to_string(v);  // integers are exact in decimal
 }
 
-// y_r += D, keeping Z = number of zero rows.
-void emit_flip_const(std::ostringstream& o, const std::vector<int64_t>& b, int n, int c, int sgn,
+// y_r += D, keeping Z = number of zero rows; wb = the walk's increments
+// (row weight x entry).
+void emit_flip_const(std::ostringstream& o, const std::vector<int64_t>& wb, int n, int c, int sgn,
                      const char* ind)
 {
     for (int r = 0; r < n; ++r) {
-        const int64_t d = 2 * b[(size_t)r * n + c] * sgn;
+        const int64_t d = wb[(size_t)r * n + c] * sgn;
         if (d == 0) continue;
         o << ind << "y" << r << " += " << lit(d) << "; Z += (y" << r << " == 0) - (y" << r << " == " << lit(d)
           << ");\n";
@@ -147,10 +148,10 @@ void emit_flip_const(std::ostringstream& o, const std::vector<int64_t>& b, int n
 }
 
 // per-lane direction: D = (d ^ lm) - lm with lm = 0 (add) or -1 (remove)
-void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& b, int n, int c, const char* ind)
+void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& wb, int n, int c, const char* ind)
 {
     for (int r = 0; r < n; ++r) {
-        const int64_t d = 2 * b[(size_t)r * n + c];
+        const int64_t d = wb[(size_t)r * n + c];
         if (d == 0) continue;
         o << ind << "{ const int D = (" << lit(d) << " ^ lm) - lm; y" << r << " += D; Z += (y" << r
           << " == 0) - (y" << r << " == D); }\n";
@@ -180,11 +181,19 @@ void emit_term(std::ostringstream& o, int n, bool negative, const char* ind)
 std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::string& name)
 {
     const int U = kGenU;
-    std::vector<int64_t> y0(n);
+    // y = 2x, halved on rows with even sums (int_row_weights): wb holds the
+    // walk's increments w_r b_rc
+    std::vector<unsigned char> w(n);
+    int_row_weights(b.data(), n, w.data());
+    std::vector<int64_t> wb(b.size()), y0(n);
     for (int i = 0; i < n; ++i) {
         int64_t rs = 0;
-        for (int j = 0; j < n; ++j) rs += b[(size_t)i * n + j];
+        for (int j = 0; j < n; ++j) {
+            rs += b[(size_t)i * n + j];
+            wb[(size_t)i * n + j] = w[i] * b[(size_t)i * n + j];
+        }
         y0[i] = 2 * b[(size_t)i * n + n - 1] - rs;
+        if (w[i] == 1) y0[i] /= 2;
     }
     std::ostringstream o;
     o << "// generated: SpaRyser kernel for one " << n << "x" << n << " integer pattern\n";
@@ -212,7 +221,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
         if (!any) continue;
         o << "        { const int mk = -(int)(((g0 >> " << j << ") & 1ull) & (unsigned)(" << j << " >= L - 1));";
         for (int r = 0; r < n; ++r) {
-            const int64_t d = 2 * b[(size_t)r * n + j];
+            const int64_t d = wb[(size_t)r * n + j];
             if (d != 0) o << " y" << r << " += " << lit(d) << " & mk;";
         }
         o << " }\n";
@@ -229,12 +238,12 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
             const int c = __builtin_ctz(j);
             if (c + 1 < U) {
                 const int sgn = ((j >> (c + 1)) & 1) ? -1 : 1;
-                emit_flip_const(o, b, n, c, sgn, ind);
+                emit_flip_const(o, wb, n, c, sgn, ind);
             } else {
                 o << ind << "if (m & 1u) {\n";
-                emit_flip_const(o, b, n, c, -1, (std::string(ind) + "    ").c_str());
+                emit_flip_const(o, wb, n, c, -1, (std::string(ind) + "    ").c_str());
                 o << ind << "} else {\n";
-                emit_flip_const(o, b, n, c, 1, (std::string(ind) + "    ").c_str());
+                emit_flip_const(o, wb, n, c, 1, (std::string(ind) + "    ").c_str());
                 o << ind << "}\n";
             }
             emit_term(o, n, (j & 1) != 0, ind);
@@ -252,11 +261,11 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
     for (int c = U; c < n - 1; ++c) {
         o << "            case " << c << ":\n"
              "                if (dir == 1) {\n";
-        emit_flip_const(o, b, n, c, 1, "                    ");
+        emit_flip_const(o, wb, n, c, 1, "                    ");
         o << "                } else if (dir == -1) {\n";
-        emit_flip_const(o, b, n, c, -1, "                    ");
+        emit_flip_const(o, wb, n, c, -1, "                    ");
         o << "                } else {\n";
-        emit_flip_lane(o, b, n, c, "                    ");
+        emit_flip_lane(o, wb, n, c, "                    ");
         o << "                }\n                break;\n";
     }
     o << "            default: break;\n            }\n";

@@ -37,15 +37,18 @@ void dense_prepare(const double* a, int n, int w, double* planes, double* x0)
 
 // Integer entry checks: |a| <= 2^30 and absolute row sums < 2^30 (so |y|
 // fits int32), y0 = y(empty set), and log2 of the bound on
-// |T| = 2^(n-1) |perm|: Bregman-Minc for 0/1 matrices, the product of
-// absolute row sums otherwise.  The mod-2^128 sum is T itself only when
-// |T| < 2^127 (checked with margin for the log arithmetic).
-int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound)
+// |T| = 2^(n-1-e) |perm| (e = rows carried halved, int_row_weights):
+// Bregman-Minc for 0/1 matrices, the product of absolute row sums
+// otherwise.  The mod-2^128 sum is T itself only when |T| < 2^127 (checked
+// with margin for the log arithmetic).
+int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound, const unsigned char* w)
 {
     if (!a || n < 1 || n > kMaxN) return SPM_EINVAL;
     bool binary = true;
     for (int e = 0; e < n * n; ++e) binary = binary && (a[e] == 0 || a[e] == 1);
-    double lb = n - 1;
+    int halved = 0;
+    for (int i = 0; w && i < n; ++i) halved += w[i] == 1 ? 1 : 0;
+    double lb = n - 1 - halved;
     for (int i = 0; i < n; ++i) {
         int64_t rs = 0, ra = 0;
         for (int j = 0; j < n; ++j) {
@@ -56,6 +59,7 @@ int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound)
         }
         if (ra >= (int64_t(1) << 30)) return SPM_ERANGE;
         y0[i] = (int)(2 * a[i * n + n - 1] - rs);
+        if (w && w[i] == 1) y0[i] /= 2;  // an even row sum: y0 is even
         if (ra == 0) lb = -1e300;  // a zero row: perm = 0
         else lb += binary ? std::lgamma((double)ra + 1.0) / ((double)ra * std::log(2.0)) : std::log2((double)ra);
     }
@@ -63,14 +67,35 @@ int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound)
     return lb >= 126.0 ? SPM_ERANGE : SPM_OK;
 }
 
-// +2A and -2A, column-major int32 (2 n^2 values).
-void int_planes(const int64_t* a, int n, int* planes)
+// Row weights of the exact walks: y_i(S) = 2 x_i(S) has the parity of the
+// row sum r_i, so for a row with an even sum every y_i is even and the walks
+// may carry y_i / 2 instead (weight 1: planes +-a_ij, y0 / 2; weight 2
+// otherwise: +-2 a_ij).  Each halved row divides T by 2, so perm =
+// (-1)^(n-1) T / 2^(n-1-e) with e halved rows, and the exact 128-bit range
+// grows by e bits (half the rows of a typical 0/1 matrix; every row of the
+// 4-per-row config-3 inputs).  Returns e.
+int int_row_weights(const int64_t* a, int n, unsigned char* w)
+{
+    int e = 0;
+    for (int i = 0; i < n; ++i) {
+        int64_t r = 0;
+        for (int j = 0; j < n; ++j) r += a[(size_t)i * n + j];
+        w[i] = (r & 1) == 0 ? 1 : 2;
+        e += w[i] == 1 ? 1 : 0;
+    }
+    return e;
+}
+
+static int64_t row_weight(const unsigned char* w, int i) { return w ? w[i] : 2; }
+
+// +wA and -wA (w = 2 unless a row is halved), column-major int32 (2 n^2 values).
+void int_planes(const int64_t* a, int n, int* planes, const unsigned char* w)
 {
     const size_t nn = (size_t)n * n;
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
-            planes[(size_t)j * n + i] = (int)(2 * a[i * n + j]);
-            planes[nn + (size_t)j * n + i] = (int)(-2 * a[i * n + j]);
+            planes[(size_t)j * n + i] = (int)(row_weight(w, i) * a[i * n + j]);
+            planes[nn + (size_t)j * n + i] = (int)(-row_weight(w, i) * a[i * n + j]);
         }
 }
 
@@ -91,15 +116,15 @@ std::vector<int64_t> ascending_col_sort(const int64_t* a, int n)
     return b;
 }
 
-// Compressed columns of +2A / -2A as the records of spm_kernels.h.
-void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob)
+// Compressed columns of +wA / -wA as the records of spm_kernels.h.
+void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob, const unsigned char* w)
 {
     std::vector<std::vector<int>> rows(n), vals(n);
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i)
             if (a[(size_t)i * n + j] != 0) {
                 rows[j].push_back(i);
-                vals[j].push_back((int)(2 * a[(size_t)i * n + j]));
+                vals[j].push_back((int)(row_weight(w, i) * a[(size_t)i * n + j]));
             }
     int extra = 0;
     for (int j = 0; j < n; ++j) {
@@ -270,14 +295,14 @@ bool int_small_ok(const int64_t* a, int n)
     return true;
 }
 
-// +2A and -2A, column-major doubles (2 n^2 values).
-void int_small_planes(const int64_t* a, int n, double* planes)
+// +wA and -wA, column-major doubles (2 n^2 values).
+void int_small_planes(const int64_t* a, int n, double* planes, const unsigned char* w)
 {
     const size_t nn = (size_t)n * n;
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
-            planes[(size_t)j * n + i] = (double)(2 * a[i * n + j]);
-            planes[nn + (size_t)j * n + i] = (double)(-2 * a[i * n + j]);
+            planes[(size_t)j * n + i] = (double)(row_weight(w, i) * a[i * n + j]);
+            planes[nn + (size_t)j * n + i] = (double)(-row_weight(w, i) * a[i * n + j]);
         }
 }
 
@@ -342,14 +367,22 @@ void f32prod_finish(int n, int log2_scale, spm_result* r)
     r->err_est = std::ldexp(r->err_est, log2_scale);
 }
 
-// T mod 2^128 -> perm = (-1)^(n-1) T / 2^(n-1).  T must be a multiple of
-// 2^(n-1); a violation means a corrupted run and is reported as a HIP error.
-int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r)
+// T mod 2^128 -> perm = (-1)^(n-1) T / 2^(n-1-e) (e halved rows; with
+// every row halved, e = n, perm = 2 (-1)^(n-1) T).  T must be a multiple of
+// 2^(n-1-e); a violation means a corrupted run and is reported as a HIP
+// error.
+int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r, int e)
 {
     const unsigned __int128 T = ((unsigned __int128)hi << 64) | lo;
-    const unsigned __int128 mask = n >= 2 ? (((unsigned __int128)1 << (n - 1)) - 1) : 0;
-    if (T & mask) return SPM_EHIP;
-    __int128 perm = ((__int128)T) >> (n - 1);
+    const int shift = n - 1 - e;
+    __int128 perm;
+    if (shift >= 0) {
+        const unsigned __int128 mask = shift >= 1 ? (((unsigned __int128)1 << shift) - 1) : 0;
+        if (T & mask) return SPM_EHIP;
+        perm = ((__int128)T) >> shift;
+    } else {
+        perm = (__int128)(T << 1);
+    }
     if ((n - 1) & 1) perm = -perm;
     r->exact_lo = (int64_t)(uint64_t)(unsigned __int128)perm;
     r->exact_hi = (int64_t)(uint64_t)(((unsigned __int128)perm) >> 64);

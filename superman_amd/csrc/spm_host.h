@@ -62,9 +62,12 @@ inline double f32prod_err_est(int n, double abs_sum)
 
 // prep.cpp -- per-matrix host preparation and result conversion
 void dense_prepare(const double* a, int n, int w, double* planes, double* x0);
-int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound);
-void int_planes(const int64_t* a, int n, int* planes);
-void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob);
+// w: per-row weights from int_row_weights (1 = row carried halved, 2 =
+// not); nullptr = every row 2 (the batch engine's layout).
+int int_row_weights(const int64_t* a, int n, unsigned char* w);
+int int_prepare(const int64_t* a, int n, int* y0, double* log2_bound, const unsigned char* w = nullptr);
+void int_planes(const int64_t* a, int n, int* planes, const unsigned char* w = nullptr);
+void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob, const unsigned char* w = nullptr);
 std::vector<int64_t> ascending_col_sort(const int64_t* a, int n);
 bool int_sparse_preferred(const int64_t* a, int n);
 double int_nonzero_term_fraction(const int64_t* a, int n);
@@ -78,12 +81,12 @@ IntWalkCosts int_walk_costs(const int64_t* a, int n);
 bool int_small_ok(const int64_t* a, int n);
 bool row_scaled_integer(const double* a, int n, std::vector<int64_t>& q, double* scale_hi, double* scale_lo);
 void scaled_int_result(double scale_hi, double scale_lo, const spm_result& t, spm_result* r);
-void int_small_planes(const int64_t* a, int n, double* planes);
+void int_small_planes(const int64_t* a, int n, double* planes, const unsigned char* w = nullptr);
 void dense_fill_result(int n, bool complex_kind, bool compensated, const double* rec, spm_result* r);
 int f32prod_scale_exponent(int n);
 int f32prod_scale(const double* a, int n, double* scaled);
 void f32prod_finish(int n, int log2_scale, spm_result* r);
-int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r);
+int int_fill_result(int n, unsigned long long lo, unsigned long long hi, spm_result* r, int halved_rows = 0);
 
 // codegen.cpp -- SpaRyser per-pattern kernels
 void ut_order_mask(const unsigned char* nz, int n, std::vector<int>& rowPerm, std::vector<int>& colPerm);

@@ -149,20 +149,24 @@ def test_auto_paths_on_small_inputs():
 
 def test_example_sparse_mtx():
     # ExampleMatrices/sparse.mtx: 45 x 45, 483 entries of 0.91 and no row or
-    # column with fewer than 5 -- one order-45 leaf.  2^44 perm(pattern) is
-    # past the exact 128-bit range, so the automatic choice takes the
-    # generated FP64 kernel.  Check: perm(0.91 P) = 0.91^45 perm(P), with
-    # perm(P) from the generated kernel of the pattern (other literals, so
-    # other roundings), within the two estimates.
+    # column with fewer than 5 -- one order-45 leaf.  20 of its rows have
+    # even sums, so the exact walks carry them halved and 2^24 perm(pattern)
+    # fits 128 bits: the automatic choice takes the exact path for the
+    # row-scaled integers, perm = 0.91^45 perm(pattern) rounded once.
+    # Checked: perm(pattern) exactly (about 45 s), against the generated
+    # FP64 kernel of the pattern (independent arithmetic, about 20 s) within
+    # its estimate.
     a = np.load(os.path.join(GOLDEN_DIR, "example_sparse.npy"), allow_pickle=False)
     with open(os.path.join(GOLDEN_DIR, "closed_form_examples.json")) as f:
         c = json.load(f)["sparse"]
-    pattern = (a != 0).astype(np.float64)
+    pattern = (a != 0).astype(np.int64)
     assert int(pattern.sum(0).min()) >= 5 and int(pattern.sum(1).min()) >= 5
-    r, path = api.permanent_auto(a)
-    assert path == "gen_real"
-    p = api.sparse_codegen_real(pattern)
-    ref = c["entry_power"] * p.value
-    assert abs(r.value - ref) <= r.err_est + c["entry_power"] * p.err_est
-    print(f"sparse.mtx: {r.value!r} ({r.seconds:.1f} s), 0.91^45 perm(P) = {ref!r}, "
-          f"cond {r.abs_sum / abs(r.value):.3g}")
+    assert int((pattern.sum(1) % 2 == 0).sum()) == 20
+    assert api.choose_path(a) == "scaled_int"
+    exact = api.int_exact(pattern).exact
+    g = api.sparse_codegen_real(pattern.astype(np.float64))
+    assert abs(g.value - exact) <= g.err_est
+    value = float(Fraction(0.91) ** 45 * exact)
+    assert abs(value / exact - c["entry_power"]) <= 1e-15 * c["entry_power"]
+    print(f"sparse.mtx: perm(pattern) = {exact} (FP64 generated kernel {g.value!r}, "
+          f"cond {g.abs_sum / abs(g.value):.3g}); perm = {value!r}")

@@ -127,3 +127,28 @@ def test_fp64_walk_sparse01_n40_matches_reference(golden):
     c = golden["c3_sparse01_n40_s40"]
     v, k = _walk(golden_matrix(c), "fp64")
     assert k == "int_small_kernel<40>" and v == int(c["ref8"])
+
+
+def _blocks_of_ones(n_block=10, blocks=4, seed=3):
+    rng = np.random.default_rng(seed)
+    n = blocks * n_block
+    j = np.zeros((n, n), dtype=np.int64)
+    for b in range(blocks):
+        j[b * n_block:(b + 1) * n_block, b * n_block:(b + 1) * n_block] = 1
+    return j[rng.permutation(n)][:, rng.permutation(n)]
+
+
+def test_halved_rows_extend_the_exact_range():
+    # four blocks of ones J_10 with rows and columns permuted (order 40):
+    # perm = (10!)^4 ~ 2^87.  With every y_i = 2 x_i carried whole, T =
+    # 2^39 perm and its Bregman bound, 2^126.2, is past the 128-bit range
+    # check; every row sum is even, so the walks carry y_i / 2 and
+    # T / 2^40 = perm / 2 fits.  Every walk and the generated kernel give
+    # the closed form (about 8 s).
+    a = _blocks_of_ones()
+    want = math.factorial(10) ** 4
+    assert O.int_log2_bound(a) >= 126.0  # the oracle's whole-y bound
+    assert api.int_exact(a).exact == want
+    for walk in ("int32", "fp64", "sparse"):
+        assert _walk(a, walk)[0] == want, walk
+    assert api.sparse_codegen(a).exact == want
