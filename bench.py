@@ -41,6 +41,17 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The generated kernel's compile time is reported as a compile: this process
+# gets a fresh, private kernel cache and no ROCm code-object cache (set
+# before the library or hiprtc is loaded).
+if "SPM_KERNEL_CACHE" not in os.environ:
+    import atexit
+    import shutil
+    import tempfile
+    os.environ["SPM_KERNEL_CACHE"] = tempfile.mkdtemp(prefix="spm_bench_kernels_")
+    atexit.register(shutil.rmtree, os.environ["SPM_KERNEL_CACHE"], True)
+os.environ.setdefault("AMD_COMGR_CACHE", "0")
+
 import numpy as np  # noqa: E402
 
 METRIC = "dense n=40 perm time & % FP64 VALU peak; batched perms/s + p99 latency"

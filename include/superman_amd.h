@@ -138,7 +138,10 @@ int spm_int_exact(const int64_t* a, int n, int device, void* stream, spm_result*
  * rebuild-and-rerun protocol (wrapper.cpp:271-344) and the driver
  * KernelGenSingleGPU (KernelGenSingleGPU.cuh:47-197).  The kernel source is
  * emitted for the matrix's pattern and exact integer values and compiled
- * in-process with hiprtc; compiled kernels are cached per (device, hash).
+ * in-process with hiprtc; compiled kernels are cached per (device, hash)
+ * in the process and as code objects on disk (directory $SPM_KERNEL_CACHE,
+ * "" or "0" = no disk cache; default $XDG_CACHE_HOME/superman_amd or
+ * ~/.cache/superman_amd), so a pattern compiles once per machine.
  * Integer matrices only, 5 <= n <= 63, same exactness bound as
  * spm_int_exact. */
 #define SPM_CG_REGISTER_EFFICIENT 1  /* rows/columns in IO::UTOrder order
@@ -218,8 +221,9 @@ int spm_plan_set_log2_chunk(spm_plan* plan, int log2_chunk);
 
 /* ---- automatic path choice ------------------------------------------------
  * One real matrix by the path whose estimated time (rates measured on
- * MI355X per Gray subset, plus the hiprtc compile of a generated kernel)
- * is smallest -- the MI355X counterpart of the reference's automatic
+ * MI355X per Gray subset, plus the hiprtc compile of a generated kernel;
+ * for an integer matrix, a load instead where that kernel is already
+ * compiled, in the process or on disk) is smallest -- the MI355X counterpart of the reference's automatic
  * algorithm choice (CommonFiles/AlgorithmSelector.h:52-67), which weighs
  * nnz per row and component size on its own GPUs:
  *   integer entries        exact walk or exact generated kernel;
@@ -227,8 +231,10 @@ int spm_plan_set_log2_chunk(spm_plan* plan, int log2_chunk);
  *                          once (err_est = 2^-52 |value|);
  *   other real entries     generated FP64 kernel or the dense FP64 walk.
  * `path_taken` (may be NULL) receives the SPM_PATH_* value that ran;
- * spm_choose_path gives the choice without running it (host only) and
- * spm_path_estimate one path's estimated seconds (compile included). */
+ * spm_choose_path gives the choice for kernels not yet compiled, without
+ * running it (host only; deterministic, so the drop-in `auto` does not
+ * depend on what a machine has compiled before), and spm_path_estimate one
+ * path's estimated seconds (compile included). */
 #define SPM_PATH_INT        1
 #define SPM_PATH_GEN_INT    2
 #define SPM_PATH_SCALED_INT 3

@@ -940,13 +940,39 @@ double est_compile_real(const Shape& s) { return 0.15 + 1.2e-3 * double(s.nnz); 
 
 }  // namespace
 
-static int choose_path(const double* a, int n, bool gen_cached, int* path)
+// Whether the register-efficient generated kernel for the integer matrix
+// `a` is already compiled (in this process or in the on-disk cache): then it
+// costs a load, not a compile.
+static bool gen_kernel_cached(const double* a, int n, bool integral, int precision, int device)
+{
+    (void)precision;
+    if (!integral) return false;
+    const int opt = SPM_CG_REGISTER_EFFICIENT;
+    std::vector<int64_t> q((size_t)n * n);
+    for (size_t e = 0; e < q.size(); ++e) q[e] = (int64_t)a[e];
+    const std::vector<int64_t> b = codegen_matrix(q.data(), n, opt);
+    const uint64_t h = pattern_hash(b.data(), n, opt);
+    const std::string name = codegen_name(h);
+    return codegen_is_cached(device, h, emit_int_kernel(b, n, name), name);
+}
+
+// A cached kernel still costs a module load (from disk: milliseconds), so
+// below order ~30, where every path takes well under a millisecond, the
+// choice never depends on what happens to be cached.  Only the exact paths
+// look at the caches (use_cache): their result is the same whichever runs,
+// while an FP64 result would change its last bits with the state of a cache.
+constexpr double kCachedLoadSeconds = 5e-3;
+
+static int choose_path(const double* a, int n, int precision, int device, bool use_cache, int* path)
 {
     const Shape s = shape_of(a, n);
     const bool can_gen = n >= kCodegenMinN;
-    const double compile = gen_cached ? 0.0 : est_compile(s);
     if (s.integral) {
-        *path = can_gen && est_gen_int(s) + compile < est_int_walk(s) ? SPM_PATH_GEN_INT : SPM_PATH_INT;
+        const double g = est_gen_int(s), w = est_int_walk(s);
+        bool gen = can_gen && g + est_compile(s) < w;
+        if (!gen && use_cache && can_gen && g + kCachedLoadSeconds < w)
+            gen = gen_kernel_cached(a, n, true, precision, device);
+        *path = gen ? SPM_PATH_GEN_INT : SPM_PATH_INT;
         return SPM_OK;
     }
     std::vector<int64_t> q;
@@ -955,8 +981,7 @@ static int choose_path(const double* a, int n, bool gen_cached, int* path)
         *path = SPM_PATH_SCALED_INT;
         return SPM_OK;
     }
-    const double compile_real = gen_cached ? 0.0 : est_compile_real(s);
-    *path = can_gen && est_gen_real(s) + compile_real < est_dense(s) ? SPM_PATH_GEN_REAL : SPM_PATH_DENSE;
+    *path = can_gen && est_gen_real(s) + est_compile_real(s) < est_dense(s) ? SPM_PATH_GEN_REAL : SPM_PATH_DENSE;
     return SPM_OK;
 }
 
@@ -965,7 +990,7 @@ extern "C" int spm_choose_path(const double* a, int n, int* path)
     if (!a || !path || n < 1 || n > kMaxN) return SPM_EINVAL;
     for (size_t e = 0; e < (size_t)n * n; ++e)
         if (!std::isfinite(a[e])) return SPM_EINVAL;
-    return choose_path(a, n, false, path);
+    return choose_path(a, n, SPM_PREC_COMPENSATED, 0, false, path);
 }
 
 extern "C" int spm_path_estimate(const double* a, int n, int path, double* seconds)
@@ -1002,7 +1027,7 @@ extern "C" int spm_permanent_auto(const double* a, int n, int precision, int dev
         if (!std::isfinite(a[e])) return out->status = SPM_EINVAL;
     const double t0 = now_seconds();
     int path = SPM_PATH_DENSE;
-    int st = choose_path(a, n, false, &path);
+    int st = choose_path(a, n, precision, device, true, &path);
     if (st != SPM_OK) return out->status = st;
     if (path == SPM_PATH_INT || path == SPM_PATH_GEN_INT) {
         std::vector<int64_t> q((size_t)n * n);
@@ -1020,7 +1045,7 @@ extern "C" int spm_permanent_auto(const double* a, int n, int precision, int dev
         row_scaled_integer(a, n, q, &sh, &sl);
         std::vector<double> qd(q.begin(), q.end());
         int qp = SPM_PATH_INT;
-        choose_path(qd.data(), n, false, &qp);
+        choose_path(qd.data(), n, precision, device, true, &qp);
         spm_result t;
         st = run_int_path(q.data(), n, qp == SPM_PATH_GEN_INT, device, stream, &t);
         if (st == SPM_OK) {

@@ -15,7 +15,8 @@
 //     variant (column order = ascending remaining in-degree, so the columns
 //     flipped most often -- column 0 every other step -- are the sparsest),
 //     or the input order for the naive variant;
-//   * y = 2x kept entirely in VGPRs (at most 63 int32 per lane: the
+//   * y = 2x (x on rows with even sums, whose y is always even) kept
+//     entirely in VGPRs (at most 63 int32 per lane: the
 //     reference's register-area cost model, UTOrderedKernelCodeGen.cuh:68-132,
 //     trades registers against global-memory x for CUDA's 255-register
 //     limit, which the 512-entry gfx950 file does not need);
@@ -28,6 +29,8 @@
 //     (a uniform switch) and whose other flips are compile-time columns;
 //   * the term prod_r y_r (128-bit) only on lanes whose zero count is 0, and
 //     128-bit accumulation (mod 2^128; the host proves |T| < 2^127).
+// Compiled code objects are kept per (device, hash) in the process and on
+// disk (codegen_get_kernel), so a pattern compiles once per machine.
 #include "superman_amd.h"
 
 #include "spm_host.h"
@@ -36,9 +39,13 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -589,6 +596,9 @@ struct CachedKernel {
 };
 std::mutex g_cg_mu;
 std::map<std::pair<int, uint64_t>, CachedKernel> g_cg_cache;
+const char* const kRtcOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+constexpr int kRtcOptCount = (int)(sizeof kRtcOpts / sizeof kRtcOpts[0]);
 }  // namespace
 
 int rtc_compile(const std::string& src, const std::string& name, std::vector<char>& code, std::string& log)
@@ -596,9 +606,7 @@ int rtc_compile(const std::string& src, const std::string& name, std::vector<cha
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), (name + ".hip").c_str(), 0, nullptr, nullptr) != HIPRTC_SUCCESS)
         return SPM_ECOMPILE;
-    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
-                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+    const hiprtcResult rc = hiprtcCompileProgram(prog, kRtcOptCount, const_cast<const char**>(kRtcOpts));
     size_t ls = 0;
     hiprtcGetProgramLogSize(prog, &ls);
     log.assign(ls, '\0');
@@ -627,6 +635,91 @@ int codegen_compile_only(const std::string& src, const std::string& name, double
     return st;
 }
 
+// ---------------------------------------------------------------------------
+// On-disk code objects.  Directory: $SPM_KERNEL_CACHE ("" or "0" turns the
+// disk cache off), else $XDG_CACHE_HOME/superman_amd, else
+// $HOME/.cache/superman_amd.  A file is <kernel name>-<FNV-1a of the source
+// and the compile options>.co, so a changed generator or option never
+// reuses an old object; it is written under a temporary name and renamed,
+// so a reader sees a whole file or none, and one that fails to load is
+// compiled again.  Any filesystem failure only means no disk cache.
+// ---------------------------------------------------------------------------
+namespace {
+
+std::string disk_cache_dir()
+{
+    const char* e = std::getenv("SPM_KERNEL_CACHE");
+    if (e) return (e[0] == 0 || (e[0] == '0' && e[1] == 0)) ? std::string() : std::string(e);
+    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && x[0]) return std::string(x) + "/superman_amd";
+    if (const char* h = std::getenv("HOME"); h && h[0]) return std::string(h) + "/.cache/superman_amd";
+    return std::string();
+}
+
+std::string disk_cache_path(const std::string& src, const std::string& name)
+{
+    const std::string dir = disk_cache_dir();
+    if (dir.empty()) return std::string();
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const char* s) {
+        for (const char* c = s; *c; ++c) {
+            h ^= (unsigned char)*c;
+            h *= 1099511628211ull;
+        }
+        h ^= 0xff;  // separator
+        h *= 1099511628211ull;
+    };
+    mix(src.c_str());
+    for (int k = 0; k < kRtcOptCount; ++k) mix(kRtcOpts[k]);
+    char tail[40];
+    std::snprintf(tail, sizeof tail, "-%016llx.co", (unsigned long long)h);
+    return dir + "/" + name + tail;
+}
+
+bool disk_load(const std::string& path, std::vector<char>& code)
+{
+    if (path.empty()) return false;
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    const long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    bool ok = sz > 0;
+    if (ok) {
+        code.resize((size_t)sz);
+        ok = std::fread(code.data(), 1, (size_t)sz, f) == (size_t)sz;
+    }
+    std::fclose(f);
+    return ok;
+}
+
+void disk_store(const std::string& path, const std::vector<char>& code)
+{
+    if (path.empty() || code.empty()) return;
+    const std::string dir = path.substr(0, path.rfind('/'));
+    for (size_t p = 1; p <= dir.size(); ++p)  // mkdir -p
+        if (p == dir.size() || dir[p] == '/') (void)::mkdir(dir.substr(0, p).c_str(), 0755);
+    char tmp[32];
+    std::snprintf(tmp, sizeof tmp, ".tmp%d", (int)::getpid());
+    const std::string t = path + tmp;
+    FILE* f = std::fopen(t.c_str(), "wb");
+    if (!f) return;
+    const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
+    if (std::fclose(f) != 0 || !ok || std::rename(t.c_str(), path.c_str()) != 0) (void)std::remove(t.c_str());
+}
+
+}  // namespace
+
+bool codegen_is_cached(int device, uint64_t hash, const std::string& src, const std::string& name)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_cg_mu);
+        if (g_cg_cache.count({device, hash})) return true;
+    }
+    const std::string path = disk_cache_path(src, name);
+    struct stat st;
+    return !path.empty() && ::stat(path.c_str(), &st) == 0 && st.st_size > 0;
+}
+
 int codegen_get_kernel(int device, uint64_t hash, const std::string& src, const std::string& name, hipFunction_t* fn,
                        double* compile_s, bool* cached)
 {
@@ -639,14 +732,29 @@ int codegen_get_kernel(int device, uint64_t hash, const std::string& src, const 
         return SPM_OK;
     }
     const double t0 = now_seconds();
+    const std::string path = disk_cache_path(src, name);
     std::vector<char> code;
+    CachedKernel ck;
+    if (disk_load(path, code) && hipModuleLoadData(&ck.module, code.data()) == hipSuccess) {
+        if (hipModuleGetFunction(&ck.fn, ck.module, name.c_str()) == hipSuccess) {
+            ck.compile_s = now_seconds() - t0;  // a load, not a compile
+            g_cg_cache[{device, hash}] = ck;
+            *fn = ck.fn;
+            *compile_s = ck.compile_s;
+            *cached = false;
+            return SPM_OK;
+        }
+        (void)hipModuleUnload(ck.module);
+        ck = CachedKernel();
+    }
+    code.clear();
     std::string log;
     int st = rtc_compile(src, name, code, log);
     if (st != SPM_OK) {
         std::fprintf(stderr, "superman_amd: hiprtc failed for %s:\n%s\n", name.c_str(), log.c_str());
         return st;
     }
-    CachedKernel ck;
+    disk_store(path, code);
     if (hipModuleLoadData(&ck.module, code.data()) != hipSuccess) return SPM_EHIP;
     if (hipModuleGetFunction(&ck.fn, ck.module, name.c_str()) != hipSuccess) return SPM_EHIP;
     ck.compile_s = now_seconds() - t0;

@@ -101,6 +101,9 @@ int codegen_unroll_bits();
 uint64_t pattern_hash(const int64_t* b, int n, int options);
 int codegen_get_kernel(int device, uint64_t hash, const std::string& src, const std::string& name, hipFunction_t* fn,
                        double* compile_s, bool* cached);
+// Whether codegen_get_kernel would find the kernel compiled (this process,
+// or the on-disk cache) -- host only.
+bool codegen_is_cached(int device, uint64_t hash, const std::string& src, const std::string& name);
 
 class DeviceGuard {
 public:

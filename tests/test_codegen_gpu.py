@@ -1,9 +1,14 @@
 """Generated SpaRyser kernels (hiprtc) on the GPU: bit-exact against the
 int128 oracle, the reference's own outputs and the generic integer kernel."""
+import json
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
-from conftest import golden_matrix, load_golden
+from conftest import ROOT, golden_matrix, load_golden
 from oracle import oracle as O
 from superman_amd import api
 from superman_amd import workloads as W
@@ -75,3 +80,38 @@ def test_order_44_triangular_closed_form():
         if above.size:
             a[i, rng.choice(above, size=min(3, above.size), replace=False)] = 1
     assert api.sparse_codegen(a).exact == 1
+
+
+def test_disk_cache_across_processes(tmp_path):
+    # a generated kernel compiled in one process is loaded from the on-disk
+    # cache by the next one: same result, a load instead of a compile
+    code = ("import json, sys; sys.path.insert(0, %r); import numpy as np; "
+            "from superman_amd import api, workloads as W; a = W.sparse01(30, 7).astype(np.int64); "
+            "p = api.CodegenPlan(a); r = p.run(); i = p.codegen_info(); p.close(); "
+            "print(json.dumps({'exact': str(r.exact), 'compile_seconds': i['compile_seconds']}))") % ROOT
+    env = dict(os.environ, SPM_KERNEL_CACHE=str(tmp_path))
+    runs = []
+    for _ in range(2):
+        p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+        assert p.returncode == 0, p.stderr
+        runs.append(json.loads(p.stdout.strip().splitlines()[-1]))
+    assert len(list(tmp_path.glob("spm_gen_*.co"))) == 1
+    assert runs[0]["exact"] == runs[1]["exact"]
+    assert runs[1]["compile_seconds"] < 0.2 * runs[0]["compile_seconds"]
+
+
+def test_auto_choice_counts_a_compiled_kernel_as_a_load(tmp_path, monkeypatch):
+    # order 40, 4 entries per row: a compile (1.5 s) does not pay for one
+    # permanent, but once the pattern's kernel is compiled the automatic
+    # path takes it (0.14 s against 0.5 s for the compressed-column walk);
+    # spm_choose_path, the choice for kernels not yet compiled, stays put.
+    # An empty on-disk cache of its own, and a pattern no other test uses.
+    monkeypatch.setenv("SPM_KERNEL_CACHE", str(tmp_path))
+    a = W.sparse01(40, 4040).astype(np.float64)
+    r0, path0 = api.permanent_auto(a)
+    assert path0 == "int"
+    with api.CodegenPlan(a.astype(np.int64)) as p:
+        assert p.run().exact == r0.exact
+    r1, path1 = api.permanent_auto(a)
+    assert path1 == "gen_int" and r1.exact == r0.exact
+    assert api.choose_path(a) == "int"
