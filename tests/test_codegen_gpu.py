@@ -84,12 +84,14 @@ def test_order_44_triangular_closed_form():
 
 def test_disk_cache_across_processes(tmp_path):
     # a generated kernel compiled in one process is loaded from the on-disk
-    # cache by the next one: same result, a load instead of a compile
+    # cache by the next one: same result, a load instead of a compile.
+    # hiprtc's own cache (comgr's) is switched off, so that the first
+    # process compiles even when an earlier test built the same source.
     code = ("import json, sys; sys.path.insert(0, %r); import numpy as np; "
             "from superman_amd import api, workloads as W; a = W.sparse01(30, 7).astype(np.int64); "
             "p = api.CodegenPlan(a); r = p.run(); i = p.codegen_info(); p.close(); "
             "print(json.dumps({'exact': str(r.exact), 'compile_seconds': i['compile_seconds']}))") % ROOT
-    env = dict(os.environ, SPM_KERNEL_CACHE=str(tmp_path))
+    env = dict(os.environ, SPM_KERNEL_CACHE=str(tmp_path), AMD_COMGR_CACHE="0")
     runs = []
     for _ in range(2):
         p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
