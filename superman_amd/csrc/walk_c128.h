@@ -40,19 +40,14 @@ constexpr int kChunk = 8;  // doubles per streamed column chunk (one 64-byte sca
 
 typedef __attribute__((address_space(4))) const double const_f64;
 
-// A wave-uniform zero the compiler cannot see through (a volatile scalar
-// move).  Offsetting a column address by it keeps that column's scalar loads
-// at their use: hoisted as loop invariants they would pin the SGPR file.
-__device__ __forceinline__ int opaque_zero()
-{
-    int z;
-    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
-    return z;
-}
-
+// A column address passed through an empty asm that claims to rewrite it:
+// the column's scalar loads stay at their use (hoisted as loop invariants
+// they would pin the SGPR file), and the asm costs no instruction.  Constant
+// offsets are added after it, so that they fold into the loads' immediates.
 __device__ __forceinline__ const const_f64* step_ptr(const double* p)
 {
-    return (const const_f64*)(p + opaque_zero());
+    asm volatile("" : "+s"(p));
+    return (const const_f64*)p;
 }
 
 __device__ __forceinline__ void fence() { __builtin_amdgcn_sched_barrier(0); }
@@ -77,7 +72,7 @@ struct Chunk {
 template <int N, int K>
 __device__ __forceinline__ void load_chunk(Chunk& c, const double* col)
 {
-    const const_f64* __restrict__ q = step_ptr(col + K * kChunk);
+    const const_f64* __restrict__ q = step_ptr(col) + K * kChunk;
 #pragma unroll
     for (int i = 0; i < kChunk; ++i)
         if (K * kChunk + i < 2 * N) c.v[i] = q[i];
@@ -132,7 +127,7 @@ template <int N, int P>
 __device__ __forceinline__ void load_part(Part<N>& c, const double* col)
 {
     constexpr int b = 2 * P * kPhaseRows<N>;
-    const const_f64* __restrict__ q = step_ptr(col + b);
+    const const_f64* __restrict__ q = step_ptr(col) + b;
 #pragma unroll
     for (int i = 0; i < 2 * kPhaseRows<N>; ++i)
         if (b + i < 2 * N) c.v[i] = q[i];

@@ -37,16 +37,14 @@ constexpr uint32_t kFoldSteps = 1;  // (no folding needed: sums are exact)
 
 typedef __attribute__((address_space(4))) const int const_i32;
 
-__device__ __forceinline__ int opaque_zero()
-{
-    int z;
-    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
-    return z;
-}
-
+// The pointer passes through an empty asm that claims to rewrite it, so a
+// step's scalar loads are neither hoisted out of the walk nor merged across
+// steps, and the asm costs no instruction (an offset from an asm would be
+// sign-extended and added in 64 bits at every load).
 __device__ __forceinline__ const const_i32* step_ptr(const int* p)
 {
-    return (const const_i32*)(p + opaque_zero());
+    asm volatile("" : "+s"(p));
+    return (const const_i32*)p;
 }
 
 __device__ __forceinline__ void fence() { __builtin_amdgcn_sched_barrier(0); }

@@ -235,17 +235,18 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Queues +/- prod_r y_r for the active lanes with no zero row.
+// Queues +/- prod_r y_r for the active lanes (the wave mask `act`) with no
+// zero row: one compare into a lane mask and a scalar test per step.
 template <int N>
 __device__ __forceinline__ void add_term(const BankA<N>& ya, const BankB<N>& yb, int Z, bool negative, bool small,
-                                         bool active, Queue& Q, uint32_t lane, unsigned __int128& acc)
+                                         uint64_t act, Queue& Q, uint32_t lane, unsigned __int128& acc)
 {
-    uint64_t need = __builtin_amdgcn_ballot_w64(active && Z == 0);
+    uint64_t need = __builtin_amdgcn_ballot_w64(Z == 0) & act;
     if constexpr (kQCap<N> == 64) {
         if (__builtin_expect(need != 0, 0)) {
             const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
             if (Q.count + cnt > kQCap<N>) flush<N>(Q, small, lane, acc);
-            if (active && Z == 0) enqueue<N>(ya, yb, negative, Q, Q.count + lanes_below(need));
+            if ((need >> lane) & 1ull) enqueue<N>(ya, yb, negative, Q, Q.count + lanes_below(need));
             Q.count += cnt;
         }
     } else {
@@ -284,6 +285,7 @@ __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const 
         for (int r = 0; r < kRowsB<N>; ++r) yb[r] = ys[kRowsA<N> + r];
     }
     const bool small = ((const const_i32*)y0)[N] != 0;
+    const uint64_t act = __builtin_amdgcn_ballot_w64(active);
     int Z = 0;
     const uint64_t k0 = h << L;
     const uint64_t g0 = k0 ^ (k0 >> 1);
@@ -300,13 +302,13 @@ __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const 
     const int lane_mask = -(int)(h & 1u);  // direction of column L-1: -1 = remove
 
     if constexpr (N == 1) {
-        add_term<N>(ya, yb, Z, false, small, active, Q, lane, acc);
+        add_term<N>(ya, yb, Z, false, small, act, Q, lane, acc);
     } else if constexpr (N == 2) {  // L = 1
-        add_term<N>(ya, yb, Z, false, small, active, Q, lane, acc);
+        add_term<N>(ya, yb, Z, false, small, act, Q, lane, acc);
         Rec r;
         load_rec(r, recs, rec_index(0, 0));
         apply_column<N, 1>(ya, yb, Z, r, recs, lane_mask);
-        add_term<N>(ya, yb, Z, true, small, active, Q, lane, acc);
+        add_term<N>(ya, yb, Z, true, small, act, Q, lane, acc);
     } else {
         const uint32_t nblk = 1u << (L - 1);
         const uint32_t half = nblk >> 1;
@@ -315,24 +317,24 @@ __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const 
             return rec_index(tz + 1, (int)((m >> (tz + 1)) & 1u));
         };
         Rec ra, rb;
-        add_term<N>(ya, yb, Z, false, small, active, Q, lane, acc);  // k = 0
+        add_term<N>(ya, yb, Z, false, small, act, Q, lane, acc);  // k = 0
         load_rec(rb, recs, rec_index(0, 0));
         apply_column<N, 0>(ya, yb, Z, rb, recs, 0);  // k = 1: +column 0
         load_rec(ra, recs, a_rec(1));
         fence();
-        add_term<N>(ya, yb, Z, true, small, active, Q, lane, acc);
+        add_term<N>(ya, yb, Z, true, small, act, Q, lane, acc);
         fence();
         auto block = [&](uint32_t m, auto SPECIAL) __attribute__((always_inline)) {
             if constexpr (decltype(SPECIAL)::value) apply_column<N, 1>(ya, yb, Z, ra, recs, lane_mask);
             else apply_column<N, 0>(ya, yb, Z, ra, recs, 0);
             load_rec(rb, recs, rec_index(0, (int)(m & 1u)));  // step 2m+1: column 0
             fence();
-            add_term<N>(ya, yb, Z, false, small, active, Q, lane, acc);
+            add_term<N>(ya, yb, Z, false, small, act, Q, lane, acc);
             fence();
             apply_column<N, 0>(ya, yb, Z, rb, recs, 0);
             load_rec(ra, recs, a_rec(m + 1));  // m+1 = nblk: column L <= N-1, a valid record
             fence();
-            add_term<N>(ya, yb, Z, true, small, active, Q, lane, acc);
+            add_term<N>(ya, yb, Z, true, small, act, Q, lane, acc);
             fence();
         };
 #pragma nounroll
