@@ -132,7 +132,6 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob, const unsi
         extra += k > kSparseSlots ? (k - 1) / kSparseSlots : 0;
     }
     const int nrec = 2 * n + 2 * extra;
-    const int bank = sparse_bank_rows(n);
     blob.assign((size_t)nrec * kSparseRec, 0);
     int next_free = 2 * n;
     for (int j = 0; j < n; ++j)
@@ -141,15 +140,12 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob, const unsi
             int rec = 2 * j + plane;
             for (int e0 = 0;; e0 += kSparseSlots) {
                 int* r = blob.data() + (size_t)rec * kSparseRec;
-                int ca = 0, cb = 0;  // bank-a slots from 0 up, bank-b slots from 6 down
-                for (int e = e0; e < std::min(k, e0 + kSparseSlots); ++e) {
-                    const int row = rows[j][e];
-                    const int v = plane ? -vals[j][e] : vals[j][e];
-                    const int s = row < bank ? ca++ : kSparseSlots - 1 - cb++;
-                    r[1 + s] = row < bank ? row : row - bank;
-                    r[1 + kSparseSlots + s] = v;
+                const int cnt = std::min(k, e0 + kSparseSlots) - e0;
+                for (int s = 0; s < cnt; ++s) {
+                    r[2 + s] = rows[j][e0 + s];
+                    r[2 + kSparseSlots + s] = plane ? -vals[j][e0 + s] : vals[j][e0 + s];
                 }
-                r[kSparseRec - 1] = ca + (cb << 16);
+                r[1] = cnt;
                 if (e0 + kSparseSlots >= k) {
                     r[0] = -1;
                     break;

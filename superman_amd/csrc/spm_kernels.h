@@ -50,23 +50,14 @@ int int_small_bits(int n);
 // formed in FP64 groups, as int_small_kernel does), else 0.  A column's
 // entries in one plane (+2A or -2A) sit in 16-int records (64 bytes, one
 // scalar load):
-//   [0] next record of the column (or -1), [1..7] rows, [8..14] values,
-//   [15] ca + 2^16 cb
-// The walk keeps rows 0..k-1 (bank a) and rows k..n-1 (bank b), k =
-// sparse_bank_rows(n), in two register arrays: a record's ca bank-a
-// entries fill slots 0, 1, ... with their row, its cb bank-b entries fill
-// slots 6, 5, ... with row - k (ca + cb <= 7; cb = 0 when n <= 32).  Record
-// 2j + p (p = 0: +2A, 1: -2A) is the first of column j; longer columns
-// continue in records after the first 2n.  Unused slots hold row 0 and
-// value 0 and are never read.
+//   [0] next record of the column (or -1), [1] entries in this record
+//   (0..7), [2..8] rows, [9..15] values
+// Record 2j + p (p = 0: +2A, 1: -2A) is the first of column j; longer
+// columns continue in records after the first 2n (a record that continues
+// is full).  Unused slots hold row 0 and value 0 and are never read.
 const void* int_sparse_kernel_ptr(int n);
 constexpr int kSparseRec = 16;   // ints per record
 constexpr int kSparseSlots = 7;  // entries per record
-// Rows in bank a: an indexed register access reaches within one tuple of
-// at most 32 registers, and the compiler indexes arrays of 8 or fewer
-// elements by compare/select chains (two per element and access) instead,
-// so past order 32 bank b gets at least 9 rows.
-constexpr int sparse_bank_rows(int n) { return n <= 32 ? n : (n - 9 < 32 ? n - 9 : 32); }
 
 // Generated SpaRyser kernels walk 2^3-step blocks with L > 3.
 constexpr int kCodegenMinN = 5;

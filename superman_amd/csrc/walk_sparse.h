@@ -12,14 +12,13 @@
 // aligned chunk of 2^L steps and all lanes flip the same column in
 // lockstep, so a column's rows and values are wave-uniform: a column comes
 // in as 64-byte records (spm_kernels.h), one scalar load each, prefetched
-// one step ahead, and a row index selects the register of y by VGPR
-// indexing (the index is an SGPR), so y stays in VGPRs.  An indexed access
-// reaches within one register tuple of at most 32 registers, so past order
-// 32 the rows are two arrays ("banks", split at sparse_bank_rows), each
-// indexed on its own; a record lists its bank-a entries from slot 0 up and
-// its bank-b entries from slot 6 down, so the bank of every slot is known
-// at compile time.  The slots in use are found by uniform tests that stop
-// at the first empty slot.  Arithmetic is exact (y = 2x in int32, T mod
+// one step ahead.  The row sums y stay in VGPRs at fixed registers, row r
+// in v(2 + r) (Rows below), and an entry of the flipped column updates its
+// row through VGPR indexing with the row as the index: one indexing region
+// reads the old value's zero test, adds in place and tests the new value
+// (row_entry), so an entry costs 3 scalar and 5 vector instructions and
+// one uniform slot test.  The slots in use are found by tests that stop at
+// the first empty slot.  Arithmetic is exact (y = 2x in int32, T mod
 // 2^128, as walk_int.h).
 #ifndef SPM_WALK_SPARSE_H
 #define SPM_WALK_SPARSE_H
@@ -50,86 +49,179 @@ __device__ __forceinline__ void load_rec(Rec& r, const int* recs, int idx)
 // The record of column j in plane p (0: +2A, 1: -2A).
 __device__ __forceinline__ int rec_index(int j, int p) { return 2 * j + p; }
 
-// Row sums of the walk: rows below sparse_bank_rows(N) in ya, the rest in
-// yb, two separate arrays (one aggregate would not be kept in registers).
-// Only compile-time indexes reach both; run-time (wave-uniform) row indexes
-// come with their bank from the record layout.
+// Row sums of the walk, pinned at v2, v3, ...: a first register tuple at
+// v2 holds rows 0..31 (fewer below order 32) and, past order 32, a second
+// one at v34 the rest, so that one wave-uniform index reaches every row
+// (row r is v(2 + r)).  Tuples come in the register-class sizes 1-12, 16
+// and 32; rows past N in the last tuple are padding, never indexed.
+constexpr int tuple_regs(int r) { return r <= 12 ? r : (r <= 16 ? 16 : 32); }
 template <int N>
-constexpr int kRowsA = sparse_bank_rows(N);
+constexpr int kTupA = tuple_regs(N < 32 ? N : 32);
 template <int N>
-constexpr int kRowsB = N - kRowsA<N>;
-template <int N>
-using BankA = int[kRowsA<N>];
-template <int N>
-using BankB = int[kRowsB<N> > 0 ? kRowsB<N> : 1];
+constexpr int kTupB = N > 32 ? tuple_regs(N - 32) : 0;
 
-// y[row] += value (MODE 0: as stored; 1: negated on lanes with m = -1,
-// the per-lane direction; 2: kept on lanes with m = -1 only, chunk-start
-// membership), keeping Z = number of zero rows.
-template <int MODE, int R>
-__device__ __forceinline__ void update_row(int (&y)[R], int& Z, int row, int v, int m)
+template <int K>
+struct Tuple {
+    typedef int T __attribute__((ext_vector_type(K > 0 ? K : 1)));
+};
+
+template <int N>
+struct Rows {
+    typename Tuple<kTupA<N>>::T a;
+    typename Tuple<kTupB<N>>::T b;  // unused up to order 32
+};
+
+template <int N, int R>
+__device__ __forceinline__ int row_get(const Rows<N>& y)
 {
-    if constexpr (MODE == 1) v = (v ^ m) - m;
-    else if constexpr (MODE == 2) v &= m;
-    const int old = y[row];
-    const int now = old + v;
-    y[row] = now;
-    Z += (now == 0) - (old == 0);
+    if constexpr (R < 32) return y.a[R];
+    else return y.b[R - 32];
 }
 
-// Slots U, U+1, ... (bank a) while U < used; nested, so a record costs one
-// scalar test per entry plus the first failing one.
-template <int U, int MODE, int R>
-__device__ __forceinline__ void slots_up(int (&y)[R], int& Z, const Rec& r, int used, int m)
+template <int N, int R>
+__device__ __forceinline__ void row_set(Rows<N>& y, int v)
 {
-    if constexpr (U < kSparseSlots) {
-        if (U < used) {
-            update_row<MODE>(y, Z, r.v[1 + U], r.v[1 + kSparseSlots + U], m);
-            slots_up<U + 1, MODE>(y, Z, r, used, m);
-        }
+    if constexpr (R < 32) y.a[R] = v;
+    else y.b[R - 32] = v;
+}
+
+template <int J, int END, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (J < END) {
+        f(std::integral_constant<int, J>{});
+        static_for<J + 1, END>(f);
     }
 }
 
-// Slots 6-U, 5-U, ... (bank b) while U < used.
-template <int U, int MODE, int R>
-__device__ __forceinline__ void slots_down(int (&y)[R], int& Z, const Rec& r, int used, int m)
-{
-    if constexpr (U < kSparseSlots) {
-        if (U < used) {
-            constexpr int s = kSparseSlots - 1 - U;
-            update_row<MODE>(y, Z, r.v[1 + s], r.v[1 + kSparseSlots + s], m);
-            slots_down<U + 1, MODE>(y, Z, r, used, m);
-        }
-    }
-}
+// y[row] += w with Z (the count of zero rows) kept: in one indexing region
+// (source 1 and destination relative to the row) the old value is tested,
+// the sum formed in place and the new value tested, into VCC and an SGPR
+// pair; Z then takes the two tests as borrow and carry.  Nothing in the
+// region writes a VGPR other than the indexed row.  Two instructions
+// separate the new value's test (a VALU write of VCC) from the carry-in
+// read of VCC, the wait gfx950 needs between them.
+// M0 is listed as clobbered (the region's index lives there); clang warns
+// that M0 is reserved, which is the point: nothing here keeps a value in it.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+#define SPM_ROW_TEXT                                        \
+    "s_set_gpr_idx_on %[row], gpr_idx(SRC1,DST)\n\t"       \
+    "v_cmp_eq_u32_e32 vcc, 0, v2\n\t"                      \
+    "s_mov_b64 %[old], vcc\n\t"                            \
+    "v_add_u32_e32 v2, %[w], v2\n\t"                       \
+    "v_cmp_eq_u32_e32 vcc, 0, v2\n\t"                      \
+    "s_set_gpr_idx_off\n\t"                                \
+    "v_subbrev_co_u32_e64 %[t], %[junk], 0, %[z], %[old]\n\t" \
+    "v_addc_co_u32_e32 %[z], vcc, 0, %[t], vcc"
+#define SPM_ROW_OUTS [z] "+v"(Z), [t] "=&v"(t), [old] "=&s"(old), [junk] "=&s"(junk)
+#define SPM_ROW_ENTRY1(CA)                                                                                  \
+    do {                                                                                                    \
+        if constexpr (VV)                                                                                   \
+            asm volatile(SPM_ROW_TEXT : "+" CA(a), SPM_ROW_OUTS : [row] "s"(row), [w] "v"(w) : "vcc", "m0"); \
+        else                                                                                                \
+            asm volatile(SPM_ROW_TEXT : "+" CA(a), SPM_ROW_OUTS : [row] "s"(row), [w] "s"(w) : "vcc", "m0"); \
+    } while (0)
+#define SPM_ROW_ENTRY2(CB)                                                                                 \
+    do {                                                                                                   \
+        if constexpr (VV)                                                                                  \
+            asm volatile(SPM_ROW_TEXT : "+{v[2:33]}"(a), "+" CB(b), SPM_ROW_OUTS : [row] "s"(row), [w] "v"(w) \
+                         : "vcc", "m0");                                                                   \
+        else                                                                                               \
+            asm volatile(SPM_ROW_TEXT : "+{v[2:33]}"(a), "+" CB(b), SPM_ROW_OUTS : [row] "s"(row), [w] "s"(w) \
+                         : "vcc", "m0");                                                                   \
+    } while (0)
 
-// The entries of one record: the count word is the bank-a count, plus the
-// bank-b count times 2^16 when N > 32.
-template <int N, int MODE>
-__device__ __forceinline__ void apply_slots(BankA<N>& ya, BankB<N>& yb, int& Z, const Rec& r, int m)
+// One row entry for each row layout: the tuples' registers are named in
+// the constraints (v2 upward, then v34 upward), which asm cannot compute.
+template <int KA, int KB, bool VV>
+__device__ __forceinline__ void row_entry(typename Tuple<KA>::T& a, typename Tuple<KB>::T& b, int& Z, int row, int w)
 {
-    const int cnt = r.v[kSparseRec - 1];
-    if constexpr (kRowsB<N> == 0) {
-        slots_up<0, MODE>(ya, Z, r, cnt, m);
+    int t;
+    uint64_t old, junk;
+    if constexpr (KB == 0) {
+        if constexpr (KA == 1) SPM_ROW_ENTRY1("{v2}");
+        else if constexpr (KA == 2) SPM_ROW_ENTRY1("{v[2:3]}");
+        else if constexpr (KA == 3) SPM_ROW_ENTRY1("{v[2:4]}");
+        else if constexpr (KA == 4) SPM_ROW_ENTRY1("{v[2:5]}");
+        else if constexpr (KA == 5) SPM_ROW_ENTRY1("{v[2:6]}");
+        else if constexpr (KA == 6) SPM_ROW_ENTRY1("{v[2:7]}");
+        else if constexpr (KA == 7) SPM_ROW_ENTRY1("{v[2:8]}");
+        else if constexpr (KA == 8) SPM_ROW_ENTRY1("{v[2:9]}");
+        else if constexpr (KA == 9) SPM_ROW_ENTRY1("{v[2:10]}");
+        else if constexpr (KA == 10) SPM_ROW_ENTRY1("{v[2:11]}");
+        else if constexpr (KA == 11) SPM_ROW_ENTRY1("{v[2:12]}");
+        else if constexpr (KA == 12) SPM_ROW_ENTRY1("{v[2:13]}");
+        else if constexpr (KA == 16) SPM_ROW_ENTRY1("{v[2:17]}");
+        else if constexpr (KA == 32) SPM_ROW_ENTRY1("{v[2:33]}");
+        else static_assert(KA < 0, "row tuple size");
     } else {
-        slots_up<0, MODE>(ya, Z, r, cnt & 0xffff, m);
-        slots_down<0, MODE>(yb, Z, r, cnt >> 16, m);
+        static_assert(KA == 32, "rows past 32 follow a full first tuple");
+        if constexpr (KB == 1) SPM_ROW_ENTRY2("{v34}");
+        else if constexpr (KB == 2) SPM_ROW_ENTRY2("{v[34:35]}");
+        else if constexpr (KB == 3) SPM_ROW_ENTRY2("{v[34:36]}");
+        else if constexpr (KB == 4) SPM_ROW_ENTRY2("{v[34:37]}");
+        else if constexpr (KB == 5) SPM_ROW_ENTRY2("{v[34:38]}");
+        else if constexpr (KB == 6) SPM_ROW_ENTRY2("{v[34:39]}");
+        else if constexpr (KB == 7) SPM_ROW_ENTRY2("{v[34:40]}");
+        else if constexpr (KB == 8) SPM_ROW_ENTRY2("{v[34:41]}");
+        else if constexpr (KB == 9) SPM_ROW_ENTRY2("{v[34:42]}");
+        else if constexpr (KB == 10) SPM_ROW_ENTRY2("{v[34:43]}");
+        else if constexpr (KB == 11) SPM_ROW_ENTRY2("{v[34:44]}");
+        else if constexpr (KB == 12) SPM_ROW_ENTRY2("{v[34:45]}");
+        else if constexpr (KB == 16) SPM_ROW_ENTRY2("{v[34:49]}");
+        else if constexpr (KB == 32) SPM_ROW_ENTRY2("{v[34:65]}");
+        else static_assert(KB < 0, "row tuple size");
+    }
+}
+
+#undef SPM_ROW_ENTRY1
+#undef SPM_ROW_ENTRY2
+#undef SPM_ROW_OUTS
+#undef SPM_ROW_TEXT
+#pragma clang diagnostic pop
+
+// One entry of a column: MODE 0 adds the record's value (an SGPR); 1 adds
+// it negated on lanes with m = -1 (the per-lane direction); 2 adds it on
+// lanes with m = -1 only (chunk-start membership).
+template <int N, int MODE>
+__device__ __forceinline__ void entry(Rows<N>& y, int& Z, int row, int v, int m)
+{
+    if constexpr (MODE == 0) row_entry<kTupA<N>, kTupB<N>, false>(y.a, y.b, Z, row, v);
+    else if constexpr (MODE == 1) row_entry<kTupA<N>, kTupB<N>, true>(y.a, y.b, Z, row, (v ^ m) - m);
+    else row_entry<kTupA<N>, kTupB<N>, true>(y.a, y.b, Z, row, v & m);
+}
+
+// Slots U, U+1, ... of a record while U < count, nested so that a record
+// costs one uniform test per entry plus the first failing one; true when
+// all slots were in use and the column continues in another record.
+template <int U, int N, int MODE>
+__device__ __forceinline__ bool apply_slots(Rows<N>& y, int& Z, const Rec& r, int cnt, int m)
+{
+    if constexpr (U == kSparseSlots) {
+        return r.v[0] >= 0;
+    } else {
+        if (U < cnt) {
+            entry<N, MODE>(y, Z, r.v[2 + U], r.v[2 + kSparseSlots + U], m);
+            return apply_slots<U + 1, N, MODE>(y, Z, r, cnt, m);
+        }
+        return false;
     }
 }
 
 // A whole column from its first record (already loaded); longer columns
 // continue in further records, loaded here (rare for sparse inputs).
 template <int N, int MODE>
-__device__ __forceinline__ void apply_column(BankA<N>& ya, BankB<N>& yb, int& Z, const Rec& first, const int* recs,
-                                             int m)
+__device__ __forceinline__ void apply_column(Rows<N>& y, int& Z, const Rec& first, const int* recs, int m)
 {
-    apply_slots<N, MODE>(ya, yb, Z, first, m);
-    int nx = first.v[0];
-    while (nx >= 0) {
-        Rec r;
-        load_rec(r, recs, nx);
-        apply_slots<N, MODE>(ya, yb, Z, r, m);
-        nx = r.v[0];
+    if (__builtin_expect(apply_slots<0, N, MODE>(y, Z, first, first.v[1], m), 0)) {
+        int nx = first.v[0];
+        for (;;) {
+            Rec r;
+            load_rec(r, recs, nx);
+            if (!apply_slots<0, N, MODE>(y, Z, r, r.v[1], m)) break;
+            nx = r.v[0];
+        }
     }
 }
 
@@ -219,14 +311,10 @@ __device__ __forceinline__ void flush(Queue& Q, bool small, uint32_t lane, unsig
 
 // Copies this lane's row sums and sign into queue entry `slot`.
 template <int N>
-__device__ __forceinline__ void enqueue(const BankA<N>& ya, const BankB<N>& yb, bool negative, Queue& Q,
-                                        uint32_t slot)
+__device__ __forceinline__ void enqueue(const Rows<N>& y, bool negative, Queue& Q, uint32_t slot)
 {
     int* e = Q.q + slot * kQStride<N>;
-#pragma unroll
-    for (int r = 0; r < kRowsA<N>; ++r) e[r] = ya[r];
-#pragma unroll
-    for (int r = 0; r < kRowsB<N>; ++r) e[kRowsA<N> + r] = yb[r];
+    static_for<0, N>([&](auto R) __attribute__((always_inline)) { e[decltype(R)::value] = row_get<N, decltype(R)::value>(y); });
     e[N] = negative ? 1 : 0;
 }
 
@@ -235,18 +323,19 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Queues +/- prod_r y_r for the active lanes (the wave mask `act`) with no
-// zero row: one compare into a lane mask and a scalar test per step.
+// Queues +/- prod_r y_r for the lanes with no zero row (a lane past the
+// matrix's last chunk carries a Z that never reaches 0): one compare into a
+// lane mask and a scalar test per step.
 template <int N>
-__device__ __forceinline__ void add_term(const BankA<N>& ya, const BankB<N>& yb, int Z, bool negative, bool small,
-                                         uint64_t act, Queue& Q, uint32_t lane, unsigned __int128& acc)
+__device__ __forceinline__ void add_term(const Rows<N>& y, int Z, bool negative, bool small, Queue& Q, uint32_t lane,
+                                         unsigned __int128& acc)
 {
-    uint64_t need = __builtin_amdgcn_ballot_w64(Z == 0) & act;
+    uint64_t need = __builtin_amdgcn_ballot_w64(Z == 0);
     if constexpr (kQCap<N> == 64) {
         if (__builtin_expect(need != 0, 0)) {
             const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
             if (Q.count + cnt > kQCap<N>) flush<N>(Q, small, lane, acc);
-            if ((need >> lane) & 1ull) enqueue<N>(ya, yb, negative, Q, Q.count + lanes_below(need));
+            if ((need >> lane) & 1ull) enqueue<N>(y, negative, Q, Q.count + lanes_below(need));
             Q.count += cnt;
         }
     } else {
@@ -258,13 +347,17 @@ __device__ __forceinline__ void add_term(const BankA<N>& ya, const BankB<N>& yb,
             uint64_t take = need;
             while ((uint32_t)__builtin_popcountll(take) > kQCap<N> - Q.count)
                 take &= ~(1ull << (63 - __builtin_clzll(take)));
-            if ((take >> lane) & 1ull) enqueue<N>(ya, yb, negative, Q, Q.count + lanes_below(take));
+            if ((take >> lane) & 1ull) enqueue<N>(y, negative, Q, Q.count + lanes_below(take));
             Q.count += (uint32_t)__builtin_popcountll(take);
             need &= ~take;
             if (need != 0) flush<N>(Q, small, lane, acc);
         }
     }
 }
+
+// Lanes past the matrix's last chunk walk a dummy chunk with this added to
+// their zero count, so that they never queue a term.
+constexpr int kInactiveZ = 1 << 16;
 
 // One lane's chunk: the 2^L Gray steps starting at k = h 2^L, its nonzero
 // terms queued (inactive lanes -- past the matrix's last chunk -- queue
@@ -275,40 +368,36 @@ template <int N>
 __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const int* __restrict__ y0, uint64_t h,
                                             int L, bool active, Queue& Q, uint32_t lane, unsigned __int128& acc)
 {
-    BankA<N> ya;
-    BankB<N> yb;
+    Rows<N> y;
     {
         const const_i32* __restrict__ ys = step_ptr(y0);
-#pragma unroll
-        for (int r = 0; r < kRowsA<N>; ++r) ya[r] = ys[r];
-#pragma unroll
-        for (int r = 0; r < kRowsB<N>; ++r) yb[r] = ys[kRowsA<N> + r];
+        static_for<0, N>([&](auto R) __attribute__((always_inline)) {
+            row_set<N, decltype(R)::value>(y, ys[decltype(R)::value]);
+        });
     }
     const bool small = ((const const_i32*)y0)[N] != 0;
-    const uint64_t act = __builtin_amdgcn_ballot_w64(active);
     int Z = 0;
     const uint64_t k0 = h << L;
     const uint64_t g0 = k0 ^ (k0 >> 1);
     for (int j = (L > 0 ? L - 1 : 0); j < N - 1; ++j) {
         Rec r;
         load_rec(r, recs, rec_index(j, 0));
-        apply_column<N, 2>(ya, yb, Z, r, recs, -(int)((g0 >> j) & 1u));
+        apply_column<N, 2>(y, Z, r, recs, -(int)((g0 >> j) & 1u));
     }
-    Z = 0;
-#pragma unroll
-    for (int r = 0; r < kRowsA<N>; ++r) Z += ya[r] == 0 ? 1 : 0;
-#pragma unroll
-    for (int r = 0; r < kRowsB<N>; ++r) Z += yb[r] == 0 ? 1 : 0;
+    Z = active ? 0 : kInactiveZ;
+    static_for<0, N>([&](auto R) __attribute__((always_inline)) {
+        Z += row_get<N, decltype(R)::value>(y) == 0 ? 1 : 0;
+    });
     const int lane_mask = -(int)(h & 1u);  // direction of column L-1: -1 = remove
 
     if constexpr (N == 1) {
-        add_term<N>(ya, yb, Z, false, small, act, Q, lane, acc);
+        add_term<N>(y, Z, false, small, Q, lane, acc);
     } else if constexpr (N == 2) {  // L = 1
-        add_term<N>(ya, yb, Z, false, small, act, Q, lane, acc);
+        add_term<N>(y, Z, false, small, Q, lane, acc);
         Rec r;
         load_rec(r, recs, rec_index(0, 0));
-        apply_column<N, 1>(ya, yb, Z, r, recs, lane_mask);
-        add_term<N>(ya, yb, Z, true, small, act, Q, lane, acc);
+        apply_column<N, 1>(y, Z, r, recs, lane_mask);
+        add_term<N>(y, Z, true, small, Q, lane, acc);
     } else {
         const uint32_t nblk = 1u << (L - 1);
         const uint32_t half = nblk >> 1;
@@ -317,24 +406,24 @@ __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const 
             return rec_index(tz + 1, (int)((m >> (tz + 1)) & 1u));
         };
         Rec ra, rb;
-        add_term<N>(ya, yb, Z, false, small, act, Q, lane, acc);  // k = 0
+        add_term<N>(y, Z, false, small, Q, lane, acc);  // k = 0
         load_rec(rb, recs, rec_index(0, 0));
-        apply_column<N, 0>(ya, yb, Z, rb, recs, 0);  // k = 1: +column 0
+        apply_column<N, 0>(y, Z, rb, recs, 0);  // k = 1: +column 0
         load_rec(ra, recs, a_rec(1));
         fence();
-        add_term<N>(ya, yb, Z, true, small, act, Q, lane, acc);
+        add_term<N>(y, Z, true, small, Q, lane, acc);
         fence();
         auto block = [&](uint32_t m, auto SPECIAL) __attribute__((always_inline)) {
-            if constexpr (decltype(SPECIAL)::value) apply_column<N, 1>(ya, yb, Z, ra, recs, lane_mask);
-            else apply_column<N, 0>(ya, yb, Z, ra, recs, 0);
+            if constexpr (decltype(SPECIAL)::value) apply_column<N, 1>(y, Z, ra, recs, lane_mask);
+            else apply_column<N, 0>(y, Z, ra, recs, 0);
             load_rec(rb, recs, rec_index(0, (int)(m & 1u)));  // step 2m+1: column 0
             fence();
-            add_term<N>(ya, yb, Z, false, small, act, Q, lane, acc);
+            add_term<N>(y, Z, false, small, Q, lane, acc);
             fence();
-            apply_column<N, 0>(ya, yb, Z, rb, recs, 0);
+            apply_column<N, 0>(y, Z, rb, recs, 0);
             load_rec(ra, recs, a_rec(m + 1));  // m+1 = nblk: column L <= N-1, a valid record
             fence();
-            add_term<N>(ya, yb, Z, true, small, act, Q, lane, acc);
+            add_term<N>(y, Z, true, small, Q, lane, acc);
             fence();
         };
 #pragma nounroll
