@@ -1,6 +1,6 @@
 """The automatic path choice (spm_choose_path, host only): MI355X's cost
 model pinned on a table of shapes, checked without a GPU.  The rows follow
-times measured on MI355X at orders 16-36 (every path, first call with the
+times measured on MI355X at orders 16-40 (every path, first call with the
 hiprtc compile and cached) and the kernels' per-subset rates at order 40."""
 import math
 
@@ -33,15 +33,19 @@ TABLE = [
     (16, 4, "int01", "int"),
     (16, 4, "real", "dense"),
     (36, 4, "real", "dense"),           # 24 ms generated + 0.35 s compile against 71 ms dense
-    # order 40, 4 per row: the compressed-column walk (0.51 s) beats a
-    # 1.5 s compile + 0.14 s generated kernel; real entries: the generated
+    # order 40, 4 per row: the compressed-column walk (0.33 s) beats a
+    # 1.5 s compile + 0.13 s generated kernel; real entries: the generated
     # FP64 kernel (0.42 s + 0.35 s compile) beats the dense walk (1.26 s)
     (40, 4, "int01", "int"),
     (40, 4, "real", "gen_real"),
     (40, 40, "real", "dense"),          # a dense pattern: suffix products do not help
-    # order 44: the generated integer kernel pays for its compile ...
+    # order 44: the generated integer kernel pays for its compile at 4
+    # entries per row ...
     (44, 4, "int01", "gen_int"),
-    (44, 8, "int01", "gen_int"),
+    # ... but not at 8: the compressed-column walk is as fast as the
+    # generated kernel there (order 40: 653 ms against 661 ms + a 2.0 s
+    # compile)
+    (44, 8, "int01", "int"),
     # ... unless every row sum is odd: no row is ever zero, every term is
     # multiplied out, and the exact FP64 walk wins
     (44, 3, "int01", "int"),
