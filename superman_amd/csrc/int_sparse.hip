@@ -1,92 +1,21 @@
-// int_sparse.hip -- single-matrix exact Ryser kernel for sparse integer
-// matrices in compressed-column form (gfx950).  The per-chunk walk is in
-// walk_sparse.h; this file holds the grid-strided task loop over one
-// matrix's chunks (each wave with its queue of deferred products in LDS),
-// the final flush of the queues and the block sum mod 2^128 into per-block
-// records, which int_dense.hip's reduce_i128_kernel sums.
-#include "walk_sparse.h"
-
-#ifndef SPM_SPARSE_ONLY_N
-#define SPM_SPARSE_ONLY_N 0
-#endif
+// int_sparse.hip -- orders 1..16 of the sparse integer kernel
+// (int_sparse.h) and the lookup of every order's kernel.
+#include "int_sparse.h"
 
 namespace spm {
-
-namespace sparse {
-
-// Orders whose queues are sized for 4 blocks per CU (kQCap) ask the
-// register allocator for the matching 4 waves per SIMD.
-template <int N>
-constexpr int kSparseWaves = kQCap<N> < 64 ? 4 : 1;
-
-template <int N>
-__global__ void __launch_bounds__(kDenseBlock) __attribute__((amdgpu_waves_per_eu(kSparseWaves<N>)))
-int_sparse_kernel(const int* __restrict__ blob,                // compressed columns (walk_sparse.h)
-                  const int* __restrict__ y0,                  // y(empty set), N values
-                  unsigned long long* __restrict__ partial,    // [gridDim.x][2]: T mod 2^128 (lo, hi)
-                  uint64_t task_begin, uint64_t task_end, uint64_t num_chunks, int L, int accumulate)
-{
-    __shared__ unsigned long long red[2 * (kDenseBlock / kWave)];
-    __shared__ int queue[(kDenseBlock / kWave) * kQCap<N> * kQStride<N>];  // per-wave product queues
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint64_t wave = (uint64_t)blockIdx.x * (kDenseBlock / kWave) + (threadIdx.x / kWave);
-    const uint64_t nwave = (uint64_t)gridDim.x * (kDenseBlock / kWave);
-    unsigned __int128 acc = 0;
-
-    Queue Q{queue + (threadIdx.x / kWave) * kQCap<N> * kQStride<N>, 0u};
-    const bool small = ((const ints::const_i32*)y0)[N] != 0;
-    for (uint64_t task = task_begin + wave; task < task_end; task += nwave) {
-        const uint64_t h = task * kWave + lane;
-        walk_sparse<N>(blob, y0, h, L, h < num_chunks, Q, lane, acc);  // lanes past the end walk a dummy chunk
-    }
-    flush<N>(Q, small, lane, acc);
-
-    // Block sum mod 2^128 (fixed order: wave butterfly, then waves in order).
-    unsigned long long lo = (unsigned long long)acc, hi = (unsigned long long)(acc >> 64);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const unsigned long long olo = __shfl_xor(lo, off, kWave);
-        const unsigned long long ohi = __shfl_xor(hi, off, kWave);
-        const unsigned long long s = lo + olo;
-        hi = hi + ohi + (s < lo ? 1ull : 0ull);
-        lo = s;
-    }
-    const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        red[2 * w] = lo;
-        red[2 * w + 1] = hi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned __int128 t = 0;
-        for (int i = 0; i < kDenseBlock / kWave; ++i) t += ((unsigned __int128)red[2 * i + 1] << 64) | red[2 * i];
-        unsigned long long* out = partial + 2 * blockIdx.x;
-        if (accumulate) t += ((unsigned __int128)out[1] << 64) | out[0];
-        out[0] = (unsigned long long)t;
-        out[1] = (unsigned long long)(t >> 64);
-    }
-}
-
-template <int N>
-const void* sparse_ptr()
-{
-    if constexpr (SPM_SPARSE_ONLY_N != 0 && N != SPM_SPARSE_ONLY_N) return nullptr;
-    else return reinterpret_cast<const void*>(&int_sparse_kernel<N>);
-}
-
-template <int... Is>
-const void* sparse_lookup(int n, std::integer_sequence<int, Is...>)
-{
-    static const void* const tab[] = {nullptr, sparse_ptr<Is + 1>()...};
-    return tab[n];
-}
-
-}  // namespace sparse
 
 const void* int_sparse_kernel_ptr(int n)
 {
     if (n < 1 || n > kMaxN) return nullptr;
-    return sparse::sparse_lookup(n, std::make_integer_sequence<int, kMaxN>{});
+    if (n <= 16) return sparse::sparse_range_ptr<1, 16>(n);
+    if (n <= 25) return int_sparse_ptr_17(n);
+    if (n <= 31) return int_sparse_ptr_26(n);
+    if (n <= 36) return int_sparse_ptr_32(n);
+    if (n <= 41) return int_sparse_ptr_37(n);
+    if (n <= 45) return int_sparse_ptr_42(n);
+    if (n <= 48) return int_sparse_ptr_46(n);
+    if (n <= 56) return int_sparse_ptr_49(n);
+    return int_sparse_ptr_57(n);
 }
 
 }  // namespace spm

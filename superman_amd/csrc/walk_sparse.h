@@ -41,7 +41,8 @@ struct Rec {
 
 __device__ __forceinline__ void load_rec(Rec& r, const int* recs, int idx)
 {
-    const const_i32* __restrict__ q = step_ptr(recs + idx * kSparseRec);
+    // the offset after the asm: a constant record index folds into the loads
+    const const_i32* __restrict__ q = step_ptr(recs) + idx * kSparseRec;
 #pragma unroll
     for (int i = 0; i < kSparseRec; ++i) r.v[i] = q[i];
 }
@@ -355,6 +356,14 @@ __device__ __forceinline__ void add_term(const Rows<N>& y, int Z, bool negative,
     }
 }
 
+// Over these orders (from where the automatic choice takes this walk to
+// where a permanent takes minutes) blocks run in (even, odd) pairs, which
+// makes column 0's record a constant and halves the loop control (352 ->
+// 296 ms on the config-3 order-40 matrix); elsewhere one block per
+// iteration keeps the kernels' code, and their compile, half as long.
+constexpr int kSparsePairMinN = 26;
+constexpr int kSparsePairMaxN = 48;
+
 // Lanes past the matrix's last chunk walk a dummy chunk with this added to
 // their zero count, so that they never queue a term.
 constexpr int kInactiveZ = 1 << 16;
@@ -413,10 +422,14 @@ __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const 
         fence();
         add_term<N>(y, Z, true, small, Q, lane, acc);
         fence();
-        auto block = [&](uint32_t m, auto SPECIAL) __attribute__((always_inline)) {
+        // Block m: step 2m flips the record in ra (column ctz(m)+1), step
+        // 2m+1 column 0 in plane m & 1 (PLANE, -1 when not known at compile
+        // time).
+        auto block = [&](uint32_t m, auto SPECIAL, auto PLANE) __attribute__((always_inline)) {
+            constexpr int plane = decltype(PLANE)::value;
             if constexpr (decltype(SPECIAL)::value) apply_column<N, 1>(y, Z, ra, recs, lane_mask);
             else apply_column<N, 0>(y, Z, ra, recs, 0);
-            load_rec(rb, recs, rec_index(0, (int)(m & 1u)));  // step 2m+1: column 0
+            load_rec(rb, recs, rec_index(0, plane >= 0 ? plane : (int)(m & 1u)));  // step 2m+1: column 0
             fence();
             add_term<N>(y, Z, false, small, Q, lane, acc);
             fence();
@@ -426,11 +439,34 @@ __device__ __forceinline__ void walk_sparse(const int* __restrict__ recs, const 
             add_term<N>(y, Z, true, small, Q, lane, acc);
             fence();
         };
+        using Even = std::integral_constant<int, 0>;
+        using Odd = std::integral_constant<int, 1>;
+        using Any = std::integral_constant<int, -1>;
+        if constexpr (N < kSparsePairMinN || N > kSparsePairMaxN) {  // one block per iteration
 #pragma nounroll
-        for (uint32_t m = 1; m < half; ++m) block(m, std::false_type{});
-        block(half, std::true_type{});
+            for (uint32_t m = 1; m < half; ++m) block(m, std::false_type{}, Any{});
+            block(half, std::true_type{}, Any{});
 #pragma nounroll
-        for (uint32_t m = half + 1; m < nblk; ++m) block(m, std::false_type{});
+            for (uint32_t m = half + 1; m < nblk; ++m) block(m, std::false_type{}, Any{});
+        } else if (L >= 3) {
+            // half = 2^(L-2) is even: blocks 1 and half+1 alone, the rest in
+            // (even, odd) pairs, so that column 0's plane is a constant
+            block(1u, std::false_type{}, Odd{});
+#pragma nounroll
+            for (uint32_t m = 2; m < half; m += 2) {
+                block(m, std::false_type{}, Even{});
+                block(m + 1, std::false_type{}, Odd{});
+            }
+            block(half, std::true_type{}, Even{});
+            block(half + 1, std::false_type{}, Odd{});
+#pragma nounroll
+            for (uint32_t m = half + 2; m < nblk; m += 2) {
+                block(m, std::false_type{}, Even{});
+                block(m + 1, std::false_type{}, Odd{});
+            }
+        } else {  // L = 2: one block, the special one
+            block(half, std::true_type{}, Any{});
+        }
     }
 }
 
