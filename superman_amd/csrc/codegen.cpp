@@ -141,27 +141,31 @@ std::string lit(int64_t v)
     return std::to_string(v);  // integers are exact in decimal
 }
 
-// y_r += D, keeping Z = number of zero rows; wb = the walk's increments
-// (row weight x entry).
-void emit_flip_const(std::ostringstream& o, const std::vector<int64_t>& wb, int n, int c, int sgn,
-                     const char* ind)
+// y_r += D, keeping Z = number of zero rows among the rows that can be zero
+// (zr[r]; a row with an odd sum never is); wb = the walk's increments (row
+// weight x entry).
+void emit_flip_const(std::ostringstream& o, const std::vector<int64_t>& wb, const std::vector<char>& zr, int n,
+                     int c, int sgn, const char* ind)
 {
     for (int r = 0; r < n; ++r) {
         const int64_t d = wb[(size_t)r * n + c] * sgn;
         if (d == 0) continue;
-        o << ind << "y" << r << " += " << lit(d) << "; Z += (y" << r << " == 0) - (y" << r << " == " << lit(d)
-          << ");\n";
+        o << ind << "y" << r << " += " << lit(d) << ";";
+        if (zr[r]) o << " Z += (y" << r << " == 0) - (y" << r << " == " << lit(d) << ");";
+        o << "\n";
     }
 }
 
 // per-lane direction: D = (d ^ lm) - lm with lm = 0 (add) or -1 (remove)
-void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& wb, int n, int c, const char* ind)
+void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& wb, const std::vector<char>& zr, int n, int c,
+                    const char* ind)
 {
     for (int r = 0; r < n; ++r) {
         const int64_t d = wb[(size_t)r * n + c];
         if (d == 0) continue;
-        o << ind << "{ const int D = (" << lit(d) << " ^ lm) - lm; y" << r << " += D; Z += (y" << r
-          << " == 0) - (y" << r << " == D); }\n";
+        o << ind << "{ const int D = (" << lit(d) << " ^ lm) - lm; y" << r << " += D;";
+        if (zr[r]) o << " Z += (y" << r << " == 0) - (y" << r << " == D);";
+        o << " }\n";
     }
 }
 
@@ -193,6 +197,9 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
     std::vector<unsigned char> w(n);
     int_row_weights(b.data(), n, w.data());
     std::vector<int64_t> wb(b.size()), y0(n);
+    // a row with an odd sum keeps y = 2x odd (2 a_{r,n-1} - r_r + 2 sum_S
+    // a_rj), so it is never zero and stays out of Z: its flips are one add
+    std::vector<char> zr(n);
     for (int i = 0; i < n; ++i) {
         int64_t rs = 0;
         for (int j = 0; j < n; ++j) {
@@ -201,6 +208,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
         }
         y0[i] = 2 * b[(size_t)i * n + n - 1] - rs;
         if (w[i] == 1) y0[i] /= 2;
+        zr[i] = (rs & 1) == 0 ? 1 : 0;
     }
     std::ostringstream o;
     o << "// generated: SpaRyser kernel for one " << n << "x" << n << " integer pattern\n";
@@ -233,8 +241,10 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
         }
         o << " }\n";
     }
-    o << "        int Z = ";
-    for (int i = 0; i < n; ++i) o << "(y" << i << " == 0)" << (i + 1 < n ? " + " : ";\n");
+    o << "        int Z = 0";
+    for (int i = 0; i < n; ++i)
+        if (zr[i]) o << " + (y" << i << " == 0)";
+    o << ";\n";
     o << "        const int lm = -(int)(h & 1ull);  // direction of column L-1: -1 = remove\n";
     o << "        const unsigned nblk = 1u << (L - " << U << ");\n"
          "        const unsigned half = nblk >> 1;\n";
@@ -245,12 +255,12 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
             const int c = __builtin_ctz(j);
             if (c + 1 < U) {
                 const int sgn = ((j >> (c + 1)) & 1) ? -1 : 1;
-                emit_flip_const(o, wb, n, c, sgn, ind);
+                emit_flip_const(o, wb, zr, n, c, sgn, ind);
             } else {
                 o << ind << "if (m & 1u) {\n";
-                emit_flip_const(o, wb, n, c, -1, (std::string(ind) + "    ").c_str());
+                emit_flip_const(o, wb, zr, n, c, -1, (std::string(ind) + "    ").c_str());
                 o << ind << "} else {\n";
-                emit_flip_const(o, wb, n, c, 1, (std::string(ind) + "    ").c_str());
+                emit_flip_const(o, wb, zr, n, c, 1, (std::string(ind) + "    ").c_str());
                 o << ind << "}\n";
             }
             emit_term(o, n, (j & 1) != 0, ind);
@@ -268,11 +278,11 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
     for (int c = U; c < n - 1; ++c) {
         o << "            case " << c << ":\n"
              "                if (dir == 1) {\n";
-        emit_flip_const(o, wb, n, c, 1, "                    ");
+        emit_flip_const(o, wb, zr, n, c, 1, "                    ");
         o << "                } else if (dir == -1) {\n";
-        emit_flip_const(o, wb, n, c, -1, "                    ");
+        emit_flip_const(o, wb, zr, n, c, -1, "                    ");
         o << "                } else {\n";
-        emit_flip_lane(o, wb, n, c, "                    ");
+        emit_flip_lane(o, wb, zr, n, c, "                    ");
         o << "                }\n                break;\n";
     }
     o << "            default: break;\n            }\n";
