@@ -18,10 +18,12 @@
 namespace spm {
 namespace sparse {
 
-// Orders whose queues are sized for 4 blocks per CU (kQCap) ask the
-// register allocator for the matching 4 waves per SIMD.
+// Orders whose queues are sized for 4 blocks per CU (kQCap), and the
+// orders with paired blocks (132-140 VGPRs at orders 26-31 otherwise, the
+// peak in the queue's flush), ask the register allocator for 4 waves per
+// SIMD.
 template <int N>
-constexpr int kSparseWaves = kQCap<N> < 64 ? 4 : 1;
+constexpr int kSparseWaves = (kQCap<N> < 64 || (N >= kSparsePairMinN && N <= kSparsePairMaxN)) ? 4 : 1;
 
 template <int N>
 __global__ void __launch_bounds__(kDenseBlock) __attribute__((amdgpu_waves_per_eu(kSparseWaves<N>)))
