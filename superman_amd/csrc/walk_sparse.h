@@ -16,7 +16,7 @@
 // in v(2 + r) (Rows below), and an entry of the flipped column updates its
 // row through VGPR indexing with the row as the index: one indexing region
 // reads the old value's zero test, adds in place and tests the new value
-// (row_entry), so an entry costs 3 scalar and 5 vector instructions and
+// (row_entry), so an entry costs 2 scalar and 5 vector instructions and
 // one uniform slot test.  The slots in use are found by tests that stop at
 // the first empty slot.  Arithmetic is exact (y = 2x in int32, T mod
 // 2^128, as walk_int.h).
@@ -96,20 +96,20 @@ __device__ __forceinline__ void static_for(F&& f)
 }
 
 // y[row] += w with Z (the count of zero rows) kept: in one indexing region
-// (source 1 and destination relative to the row) the old value is tested,
-// the sum formed in place and the new value tested, into VCC and an SGPR
-// pair; Z then takes the two tests as borrow and carry.  Nothing in the
-// region writes a VGPR other than the indexed row.  Two instructions
-// separate the new value's test (a VALU write of VCC) from the carry-in
-// read of VCC, the wait gfx950 needs between them.
+// (source 1 and destination relative to the row) the old value is tested
+// into an SGPR pair, the sum formed in place and the new value tested into
+// VCC; Z then takes the two tests as borrow and carry.  Nothing in the
+// region writes a VGPR other than the indexed row, and a compare's SGPR
+// destination is not relative to the index (checked on gfx950).  At least
+// two instructions separate each VALU write of an SGPR pair from its read
+// as carry-in, the wait gfx950 needs between them.
 // M0 is listed as clobbered (the region's index lives there); clang warns
 // that M0 is reserved, which is the point: nothing here keeps a value in it.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
 #define SPM_ROW_TEXT                                        \
     "s_set_gpr_idx_on %[row], gpr_idx(SRC1,DST)\n\t"       \
-    "v_cmp_eq_u32_e32 vcc, 0, v2\n\t"                      \
-    "s_mov_b64 %[old], vcc\n\t"                            \
+    "v_cmp_eq_u32_e64 %[old], 0, v2\n\t"                   \
     "v_add_u32_e32 v2, %[w], v2\n\t"                       \
     "v_cmp_eq_u32_e32 vcc, 0, v2\n\t"                      \
     "s_set_gpr_idx_off\n\t"                                \
