@@ -917,7 +917,10 @@ double per_row(const Shape& s) { return double(s.nnz) / s.n; }
 // row 20 ms, dense 295 ms against 71 ms for the dense walk), and for
 // integer kernels the 128-bit product on the fraction q of terms with no
 // zero row (order 36, 3 entries per row, odd row sums: 291 ms; 4 per row:
-// 9 ms); compiles from the first-call minus cached times.
+// 9 ms; the integer kernels' per-subset rate grows as (entries per row)^2.5:
+// at orders 36 and 40, 2.4-2.7e-13 s with 4 entries per row, 6.1-7.8e-13
+// with 6, 1.2-1.8e-12 with 8); compiles from the first-call minus cached
+// times.
 double est_dense(const Shape& s) { return 3e-5 + subsets(s.n) * 2.0 * s.n / 3.5e13; }
 double est_int_walk(const Shape& s)
 {
@@ -928,7 +931,7 @@ double est_int_walk(const Shape& s)
 double est_gen_int(const Shape& s)
 {
     const double pr = per_row(s);
-    return 3e-5 + subsets(s.n) * (2.0e-14 * pr * pr + 2.3e-13 * s.n * s.walks.q);
+    return 3e-5 + subsets(s.n) * (8.0e-15 * std::pow(pr, 2.5) + 2.3e-13 * s.n * s.walks.q);
 }
 double est_gen_real(const Shape& s)
 {

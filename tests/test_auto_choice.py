@@ -33,7 +33,7 @@ TABLE = [
     (16, 4, "int01", "int"),
     (16, 4, "real", "dense"),
     (36, 4, "real", "dense"),           # 24 ms generated + 0.35 s compile against 71 ms dense
-    # order 40, 4 per row: the compressed-column walk (0.33 s) beats a
+    # order 40, 4 per row: the compressed-column walk (0.25 s) beats a
     # 1.5 s compile + 0.13 s generated kernel; real entries: the generated
     # FP64 kernel (0.42 s + 0.35 s compile) beats the dense walk (1.26 s)
     (40, 4, "int01", "int"),
@@ -42,8 +42,8 @@ TABLE = [
     # order 44: the generated integer kernel pays for its compile at 4
     # entries per row ...
     (44, 4, "int01", "gen_int"),
-    # ... but not at 8: the compressed-column walk is as fast as the
-    # generated kernel there (order 40: 653 ms against 661 ms + a 2.0 s
+    # ... but not at 8: the compressed-column walk is faster than the
+    # generated kernel there (order 40: 565 ms against 661 ms + a 2.0 s
     # compile)
     (44, 8, "int01", "int"),
     # ... unless every row sum is odd: no row is ever zero, every term is
