@@ -35,7 +35,7 @@ def run(a, walk):
         return p.run().exact
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 12, 17, 20, 24])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 17, 20, 24])
 def test_vs_oracle(n):
     for seed, vmax, signed in ((n, 1, False), (100 + n, 3, True)):
         a = sparse_ints(n, seed, per_row=4, vmax=vmax, signed=signed)
@@ -101,22 +101,26 @@ def fib(k):
 def test_tridiagonal_fibonacci(n):
     # perm of the 0/1 tridiagonal matrix: p(n) = p(n-1) + p(n-2), p(1) = 1,
     # p(2) = 2, so F(n+1).  Past order 32 the rows sit in two register
-    # banks and the columns at the split have entries in both.
+    # tuples and the columns at the split have entries in both.
     a = np.eye(n, dtype=np.int64) + np.eye(n, k=1, dtype=np.int64) + np.eye(n, k=-1, dtype=np.int64)
     assert run(a, "sparse") == fib(n + 1)
 
 
-@pytest.mark.parametrize("n", [33, 35])
+@pytest.mark.parametrize("n", [33, 35, 37, 38, 42, 43, 44, 45])
 def test_identity_plus_cycle(n):
-    # only the identity and the n-cycle avoid the zeros of I + P
+    # only the identity and the n-cycle avoid the zeros of I + P.  The
+    # orders cover every size of the second register tuple that a walk of
+    # a few seconds reaches (rows past 32: 1-12, then 16 at orders 45-48;
+    # order 45 takes about 4 s); orders 49-63 (a 32-register second tuple)
+    # build but take over a minute per walk and are not run here
     a = np.eye(n, dtype=np.int64) + np.roll(np.eye(n, dtype=np.int64), 1, axis=1)
     assert run(a, "sparse") == 2
 
 
 @pytest.mark.parametrize("n", [33, 34, 36])
-def test_both_banks_long_columns(n):
+def test_both_tuples_long_columns(n):
     # signed entries, two 12-entry columns (continuation records with rows in
-    # both banks); the int32 dense walk and a row-permuted copy must agree
+    # both register tuples); the int32 dense walk and a row-permuted copy must agree
     a = sparse_ints(n, 500 + n, per_row=4, vmax=3, signed=True)
     rng = np.random.default_rng(n)
     for j in (0, n - 1):
