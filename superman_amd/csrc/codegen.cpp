@@ -114,7 +114,18 @@ std::vector<int64_t> apply_perm(const int64_t* a, int n, const std::vector<int>&
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int kGenU = 3;  // log2 of the steps per generated block
+// log2 of the steps per generated block.  A block's first flip goes
+// through a chain of bit tests on the block index, its 2^U - 1 others are
+// straight-line code with the column and direction known at compile time,
+// so longer blocks run faster and compile longer (2^U copies of the term
+// product; about 2.5e-3 2^U n seconds of hiprtc).  Measured on order-40 0/1
+// patterns (config 3 / rows of 4 and 5 entries): U = 3 127.6 / 198.1 ms,
+// 0.9 s compile; U = 4 100.0 / 187.1 ms, 1.6 s; U = 5 77.1 / 175.8 ms,
+// 3.3 s.  Integer kernels: U = 3 up to order 30 (a permanent there takes
+// milliseconds, the compile dominates), 4 up to 45, 5 beyond (one walk takes
+// tens of seconds); real kernels keep 3.
+int gen_unroll_bits_int(int n) { return n <= 30 ? 3 : (n <= 45 ? 4 : 5); }
+constexpr int kGenURealBits = 3;
 
 // Waves per SIMD asked of the register allocator (0: the compiler's
 // choice).  Orders 29..40: the natural allocation (100-127 VGPRs) leaves 4;
@@ -156,14 +167,15 @@ void emit_flip_const(std::ostringstream& o, const std::vector<int64_t>& wb, cons
     }
 }
 
-// per-lane direction: D = (d ^ lm) - lm with lm = 0 (add) or -1 (remove)
+// per-lane direction: D = (d ^ mask) - mask with mask = 0 (add) or -1
+// (remove) in each lane
 void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& wb, const std::vector<char>& zr, int n, int c,
-                    const char* ind)
+                    const char* mask, const char* ind)
 {
     for (int r = 0; r < n; ++r) {
         const int64_t d = wb[(size_t)r * n + c];
         if (d == 0) continue;
-        o << ind << "{ const int D = (" << lit(d) << " ^ lm) - lm; y" << r << " += D;";
+        o << ind << "{ const int D = (" << lit(d) << " ^ " << mask << ") - " << mask << "; y" << r << " += D;";
         if (zr[r]) o << " Z += (y" << r << " == 0) - (y" << r << " == D);";
         o << " }\n";
     }
@@ -191,7 +203,7 @@ void emit_term(std::ostringstream& o, int n, bool negative, const char* ind)
 
 std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::string& name)
 {
-    const int U = kGenU;
+    const int U = gen_unroll_bits_int(n);
     // y = 2x, halved on rows with even sums (int_row_weights): wb holds the
     // walk's increments w_r b_rc
     std::vector<unsigned char> w(n);
@@ -270,22 +282,17 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
     emit_term(o, n, false, "            ");
     block_tail("            ");
     o << "        }\n";
+    // block m first flips column U + ctz(m), in the block's direction
+    // (bit ctz(m)+1 of m) or, at the middle block, each lane's: one mask,
+    // so each column's flip is one branch-free path, found by testing the
+    // bits of m from the lowest (half the blocks stop at the first test)
     o << "        for (unsigned m = 1; m < nblk; ++m) {\n"
-         "            const int tz = __builtin_ctz(m);\n"
-         "            const int c = tz + " << U << ";\n"
-         "            const int dir = (m == half) ? 0 : (((m >> (tz + 1)) & 1u) ? -1 : 1);\n"
-         "            switch (c) {\n";
-    for (int c = U; c < n - 1; ++c) {
-        o << "            case " << c << ":\n"
-             "                if (dir == 1) {\n";
-        emit_flip_const(o, wb, zr, n, c, 1, "                    ");
-        o << "                } else if (dir == -1) {\n";
-        emit_flip_const(o, wb, zr, n, c, -1, "                    ");
-        o << "                } else {\n";
-        emit_flip_lane(o, wb, zr, n, c, "                    ");
-        o << "                }\n                break;\n";
+         "            const int lmm = (m == half) ? lm : -(int)((m >> (__builtin_ctz(m) + 1)) & 1u);\n";
+    for (int c = U; c < n - 1 && c - U < 32; ++c) {
+        o << "            " << (c == U ? "if" : "} else if") << " (m & " << (1u << (c - U)) << "u) {\n";
+        emit_flip_lane(o, wb, zr, n, c, "lmm", "                ");
     }
-    o << "            default: break;\n            }\n";
+    if (U < n - 1) o << "            }\n";
     emit_term(o, n, false, "            ");
     block_tail("            ");
     o << "        }\n";
@@ -399,13 +406,16 @@ std::string prod_expr(int lo, int hi)
     return e;
 }
 
-void emit_real_flip(std::ostringstream& o, const std::vector<double>& b, int n, int c, int sgn, const char* ind)
+// sgn 0: the per-lane sign in the variable `sign` (x += sign * d, exact for
+// sign = +-1)
+void emit_real_flip(std::ostringstream& o, const std::vector<double>& b, int n, int c, int sgn, const char* ind,
+                    const char* sign = "sd")
 {
     for (int r = 0; r < n; ++r) {
         const double d = b[(size_t)r * n + c];
         if (d == 0.0) continue;
         if (sgn == 0)
-            o << ind << "x" << r << " = __builtin_fma(sd, " << hexlit(d) << ", x" << r << ");\n";
+            o << ind << "x" << r << " = __builtin_fma(" << sign << ", " << hexlit(d) << ", x" << r << ");\n";
         else
             o << ind << "x" << r << " += " << hexlit(sgn > 0 ? d : -d) << ";\n";
     }
@@ -433,7 +443,7 @@ void emit_real_term(std::ostringstream& o, const RealLevels& lv, bool negative, 
 
 std::string emit_real_kernel(const std::vector<double>& b, int n, bool compensated, const std::string& name)
 {
-    const int U = kGenU;
+    const int U = kGenURealBits;
     const RealLevels lv = real_levels(b, n);
     std::vector<double> x0(n);
     for (int i = 0; i < n; ++i) {  // x(empty set) with the left-to-right row sum (dpSingleGPU.cuh:62-74)
@@ -511,22 +521,15 @@ std::string emit_real_kernel(const std::vector<double>& b, int n, bool compensat
     emit_real_term(o, lv, false, "            ");
     block_tail("            ");
     o << "        }\n";
+    // as in the integer kernels: one per-lane sign and a chain of bit tests
     o << "        for (unsigned m = 1; m < nblk; ++m) {\n"
-         "            const int tz = __builtin_ctz(m);\n"
-         "            const int c = tz + " << U << ";\n"
-         "            const int dir = (m == half) ? 0 : (((m >> (tz + 1)) & 1u) ? -1 : 1);\n"
-         "            switch (c) {\n";
-    for (int c = U; c < n - 1; ++c) {
-        o << "            case " << c << ":\n"
-             "                if (dir == 1) {\n";
-        flip(c, 1, "                    ");
-        o << "                } else if (dir == -1) {\n";
-        flip(c, -1, "                    ");
-        o << "                } else {\n";
-        flip(c, 0, "                    ");
-        o << "                }\n                break;\n";
+         "            const double sdm = (m == half) ? sd : (((m >> (__builtin_ctz(m) + 1)) & 1u) ? -1.0 : 1.0);\n";
+    for (int c = U; c < n - 1 && c - U < 32; ++c) {
+        o << "            " << (c == U ? "if" : "} else if") << " (m & " << (1u << (c - U)) << "u) {\n";
+        emit_real_flip(o, b, n, c, 0, "                ", "sdm");
+        if (c > 0) emit_real_levels(o, lv, n, c, "                ");
     }
-    o << "            default: break;\n            }\n";
+    if (U < n - 1) o << "            }\n";
     emit_real_term(o, lv, false, "            ");
     block_tail("            ");
     o << "        }\n";
@@ -567,7 +570,7 @@ uint64_t real_pattern_hash(const double* b, int n, int options, int precision)
     mix((uint64_t)n);
     mix((uint64_t)options);
     mix((uint64_t)precision);
-    mix((uint64_t)kGenU);
+    mix((uint64_t)kGenURealBits);
     for (int e = 0; e < n * n; ++e) {
         uint64_t w;
         std::memcpy(&w, b + e, 8);
@@ -576,7 +579,7 @@ uint64_t real_pattern_hash(const double* b, int n, int options, int precision)
     return h;
 }
 
-int codegen_unroll_bits() { return kGenU; }
+int codegen_unroll_bits(int n, bool integral) { return integral ? gen_unroll_bits_int(n) : kGenURealBits; }
 
 // FNV-1a over the pattern, values and generator options.
 uint64_t pattern_hash(const int64_t* b, int n, int options)
@@ -590,7 +593,7 @@ uint64_t pattern_hash(const int64_t* b, int n, int options)
     };
     mix((uint64_t)n);
     mix((uint64_t)options);
-    mix((uint64_t)kGenU);
+    mix((uint64_t)gen_unroll_bits_int(n));
     for (int e = 0; e < n * n; ++e) mix((uint64_t)b[e]);
     return h;
 }

@@ -97,7 +97,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
 std::string emit_real_kernel(const std::vector<double>& b, int n, bool compensated, const std::string& name);
 int real_levels_count(const std::vector<double>& b, int n);
 uint64_t real_pattern_hash(const double* b, int n, int options, int precision);
-int codegen_unroll_bits();
+int codegen_unroll_bits(int n, bool integral);  // log2 of a generated kernel's block length
 uint64_t pattern_hash(const int64_t* b, int n, int options);
 int codegen_get_kernel(int device, uint64_t hash, const std::string& src, const std::string& name, hipFunction_t* fn,
                        double* compile_s, bool* cached);
