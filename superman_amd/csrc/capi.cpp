@@ -917,10 +917,13 @@ double per_row(const Shape& s) { return double(s.nnz) / s.n; }
 // row 20 ms, dense 295 ms against 71 ms for the dense walk), and for
 // integer kernels the 128-bit product on the fraction q of terms with no
 // zero row (order 36, 3 entries per row, odd row sums: 291 ms; 4 per row:
-// 9 ms; the integer kernels' per-subset rate grows as (entries per row)^2.5:
-// at orders 36 and 40, 2.4-2.7e-13 s with 4 entries per row, 6.1-7.8e-13
-// with 6, 1.2-1.8e-12 with 8); compiles from the first-call minus cached
-// times.
+// 9 ms).  Per-subset rates of the current generated kernels at orders 36
+// and 40 (0/1 and N(0,1) values on 4, 6 and 8 entries per row): integer
+// 1.6-2.3e-13 s, 4.9-6.7e-13, 1.1-1.6e-12 (as (entries per row)^2.75);
+// real 1.6-1.8e-13, 2.4-2.8e-13, 3.5-3.8e-13 (the quadratic term keeps
+// dense patterns on the dense walk).  Compiles: first-call minus cached
+// times; integer kernels about 2.5e-3 2^U n s (U = log2 of the block length
+// for order n), real ones 0.25-0.35 s.
 double est_dense(const Shape& s) { return 3e-5 + subsets(s.n) * 2.0 * s.n / 3.5e13; }
 double est_int_walk(const Shape& s)
 {
@@ -931,12 +934,12 @@ double est_int_walk(const Shape& s)
 double est_gen_int(const Shape& s)
 {
     const double pr = per_row(s);
-    return 3e-5 + subsets(s.n) * (8.0e-15 * std::pow(pr, 2.5) + 2.3e-13 * s.n * s.walks.q);
+    return 3e-5 + subsets(s.n) * (4.3e-15 * std::pow(pr, 2.75) + 2.3e-13 * s.n * s.walks.q);
 }
 double est_gen_real(const Shape& s)
 {
     const double pr = per_row(s);
-    return 3e-5 + subsets(s.n) * ((pr + 14.0) * 3.8e-14 + 5e-15 * pr * pr);
+    return 3e-5 + subsets(s.n) * (3.0e-14 * pr + 4e-15 * pr * pr);
 }
 double est_compile(const Shape& s) { return 0.1 + 2.5e-3 * std::ldexp(1.0, codegen_unroll_bits(s.n, true)) * s.n; }
 double est_compile_real(const Shape& s) { return 0.15 + 1.2e-3 * double(s.nnz); }

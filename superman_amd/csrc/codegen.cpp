@@ -123,9 +123,12 @@ namespace {
 // 0.9 s compile; U = 4 100.0 / 187.1 ms, 1.6 s; U = 5 77.1 / 175.8 ms,
 // 3.3 s.  Integer kernels: U = 3 up to order 30 (a permanent there takes
 // milliseconds, the compile dominates), 4 up to 45, 5 beyond (one walk takes
-// tens of seconds); real kernels keep 3.
+// tens of seconds).  Real kernels form their term from cached suffix
+// products, so their code stays small: U = 5 from order 21 (order 40, N(0,1)
+// on 4 entries per row: 136 / 102 / 85 ms at U = 3 / 4 / 5, 0.25 s compile
+// each).
 int gen_unroll_bits_int(int n) { return n <= 30 ? 3 : (n <= 45 ? 4 : 5); }
-constexpr int kGenURealBits = 3;
+int gen_unroll_bits_real(int n) { return n <= 20 ? 3 : 5; }
 
 // Waves per SIMD asked of the register allocator (0: the compiler's
 // choice).  Orders 29..40: the natural allocation (100-127 VGPRs) leaves 4;
@@ -443,7 +446,7 @@ void emit_real_term(std::ostringstream& o, const RealLevels& lv, bool negative, 
 
 std::string emit_real_kernel(const std::vector<double>& b, int n, bool compensated, const std::string& name)
 {
-    const int U = kGenURealBits;
+    const int U = gen_unroll_bits_real(n);
     const RealLevels lv = real_levels(b, n);
     std::vector<double> x0(n);
     for (int i = 0; i < n; ++i) {  // x(empty set) with the left-to-right row sum (dpSingleGPU.cuh:62-74)
@@ -570,7 +573,7 @@ uint64_t real_pattern_hash(const double* b, int n, int options, int precision)
     mix((uint64_t)n);
     mix((uint64_t)options);
     mix((uint64_t)precision);
-    mix((uint64_t)kGenURealBits);
+    mix((uint64_t)gen_unroll_bits_real(n));
     for (int e = 0; e < n * n; ++e) {
         uint64_t w;
         std::memcpy(&w, b + e, 8);
@@ -579,7 +582,7 @@ uint64_t real_pattern_hash(const double* b, int n, int options, int precision)
     return h;
 }
 
-int codegen_unroll_bits(int n, bool integral) { return integral ? gen_unroll_bits_int(n) : kGenURealBits; }
+int codegen_unroll_bits(int n, bool integral) { return integral ? gen_unroll_bits_int(n) : gen_unroll_bits_real(n); }
 
 // FNV-1a over the pattern, values and generator options.
 uint64_t pattern_hash(const int64_t* b, int n, int options)

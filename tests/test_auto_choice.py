@@ -32,10 +32,10 @@ TABLE = [
     # below order ~30 a call is host-overhead bound: no compile pays
     (16, 4, "int01", "int"),
     (16, 4, "real", "dense"),
-    (36, 4, "real", "dense"),           # 24 ms generated + 0.35 s compile against 71 ms dense
+    (36, 4, "real", "dense"),           # 6 ms generated + 0.24 s compile against 71 ms dense
     # order 40, 4 per row: the compressed-column walk (0.25 s) beats a
-    # 1.5 s compile + 0.13 s generated kernel; real entries: the generated
-    # FP64 kernel (0.42 s + 0.35 s compile) beats the dense walk (1.26 s)
+    # 1.6 s compile + 0.09 s generated kernel; real entries: the generated
+    # FP64 kernel (0.09 s + 0.25 s compile) beats the dense walk (1.26 s)
     (40, 4, "int01", "int"),
     (40, 4, "real", "gen_real"),
     (40, 40, "real", "dense"),          # a dense pattern: suffix products do not help
