@@ -13,9 +13,10 @@
 // own line.  Keys as the reference's: filename, algorithm, mode (single_gpu
 // runs; cpu and the multi-GPU modes end with an error), calculation_precision
 // (dd | kahan; a number selects the reference's CPU-only arbitrary
-// precision, an error here), complex, binary, undirected, device_id; pid,
-// repo_dir, thread_count, gpu_num and the approximation's scaling keys are
-// read and have no effect.
+// precision, an error here), complex, binary, undirected, device_id,
+// scaling, scaling_iteration_no and scale_into (every leaf balanced before
+// its permanent, as DecomposePerman's scaling option); pid, repo_dir,
+// thread_count and gpu_num are read and have no effect.
 #include "superman/AlgorithmSelector.h"
 
 #include <chrono>
