@@ -145,7 +145,7 @@ void int_sparse_blob(const int64_t* a, int n, std::vector<int>& blob, const unsi
                     r[2 + s] = rows[j][e0 + s];
                     r[2 + kSparseSlots + s] = plane ? -vals[j][e0 + s] : vals[j][e0 + s];
                 }
-                r[1] = cnt;
+                r[1] = std::max(cnt, 1);  // a zero column: one entry of value 0 on row 0 (slots are zeroed)
                 if (e0 + kSparseSlots >= k) {
                     r[0] = -1;
                     break;

@@ -51,10 +51,12 @@ int int_small_bits(int n);
 // entries in one plane (+2A or -2A) sit in 16-int records (64 bytes, one
 // scalar load):
 //   [0] next record of the column (or -1), [1] entries in this record
-//   (0..7), [2..8] rows, [9..15] values
+//   (1..7), [2..8] rows, [9..15] values
 // Record 2j + p (p = 0: +2A, 1: -2A) is the first of column j; longer
 // columns continue in records after the first 2n (a record that continues
-// is full).  Unused slots hold row 0 and value 0 and are never read.
+// is full).  A zero column has one entry of value 0 on row 0, so that every
+// record's first slot is in use.  Unused slots hold row 0 and value 0 and
+// are never read.
 const void* int_sparse_kernel_ptr(int n);
 constexpr int kSparseRec = 16;   // ints per record
 constexpr int kSparseSlots = 7;  // entries per record

@@ -194,15 +194,16 @@ __device__ __forceinline__ void entry(Rows<N>& y, int& Z, int row, int v, int m)
 }
 
 // Slots U, U+1, ... of a record while U < count, nested so that a record
-// costs one uniform test per entry plus the first failing one; true when
-// all slots were in use and the column continues in another record.
+// costs one uniform test per entry after the first (slot 0 is always in
+// use) plus the first failing one; true when all slots were in use and the
+// column continues in another record.
 template <int U, int N, int MODE>
 __device__ __forceinline__ bool apply_slots(Rows<N>& y, int& Z, const Rec& r, int cnt, int m)
 {
     if constexpr (U == kSparseSlots) {
         return r.v[0] >= 0;
     } else {
-        if (U < cnt) {
+        if (U == 0 || U < cnt) {
             entry<N, MODE>(y, Z, r.v[2 + U], r.v[2 + kSparseSlots + U], m);
             return apply_slots<U + 1, N, MODE>(y, Z, r, cnt, m);
         }
@@ -331,9 +332,14 @@ template <int N>
 __device__ __forceinline__ void add_term(const Rows<N>& y, int Z, bool negative, bool small, Queue& Q, uint32_t lane,
                                          unsigned __int128& acc)
 {
-    uint64_t need = __builtin_amdgcn_ballot_w64(Z == 0);
+    // The step's test branches on a ballot whose mask is not used again, so
+    // the compare goes straight to the branch; the rare taken branch takes
+    // the mask anew (Z <= 0 names the same lanes, Z never being negative,
+    // and is not merged with the test): 249.5 -> 228.4 ms at order 40.
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(Z == 0) == 0, 1)) return;
+    uint64_t need = __builtin_amdgcn_ballot_w64(Z <= 0);
     if constexpr (kQCap<N> == 64) {
-        if (__builtin_expect(need != 0, 0)) {
+        {
             const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
             if (Q.count + cnt > kQCap<N>) flush<N>(Q, small, lane, acc);
             if ((need >> lane) & 1ull) enqueue<N>(y, negative, Q, Q.count + lanes_below(need));
