@@ -183,7 +183,7 @@ double int_nonzero_term_fraction(const int64_t* a, int n)
 
 // Seconds per Gray subset of the exact integer walks on MI355X, fitted to
 // measured kernel times on 0/1 matrices of orders 28-40 with 3, 4, 6 and 8
-// entries per row (within 35 %, 22 % from order 36).  Every walk pays, on
+// entries per row (within 46 %, 32 % from order 36).  Every walk pays, on
 // the fraction q of terms with no zero row, for the 128-bit product:
 //  - compressed-column walk: per entry of the flipped column (p = entries
 //    per row), more at small orders where chunks are short and their
@@ -191,8 +191,8 @@ double int_nonzero_term_fraction(const int64_t* a, int n)
 //    orders 39-41, whose queues are capped below a wave's 64 lanes;
 //  - FP64 walk (every absolute row sum below 2^b) and int32 walk: per row.
 // With odd row sums (q = 1) the sparse walk loses its advantage: sparse
-// 0/1 order 40 with 3 entries per row takes 7.5 s there against 2.5 s in
-// the FP64 walk, and 0.25 s with 4 entries per row (q near 0).
+// 0/1 order 40 with 3 entries per row takes 7.4 s there against 2.5 s in
+// the FP64 walk, and 0.21 s with 4 entries per row (q near 0).
 IntWalkCosts int_walk_costs(const int64_t* a, int n)
 {
     IntWalkCosts c;
@@ -201,7 +201,7 @@ IntWalkCosts int_walk_costs(const int64_t* a, int n)
     c.per_row = double(nnz) / n;
     c.q = int_nonzero_term_fraction(a, n);
     const double capped = (n >= 39 && n <= 41) ? 2.0 : 1.0;
-    c.sparse = 1.25e-13 * c.per_row * (1.0 + 1.18 * std::ldexp(1.0, 28 - n)) + 1.69e-13 * capped * n * c.q;
+    c.sparse = 1.13e-13 * c.per_row * (1.0 + 1.44 * std::ldexp(1.0, 28 - n)) + 1.72e-13 * capped * n * c.q;
     c.fp64 = int_small_ok(a, n) ? 6.75e-14 * n + 4.7e-14 * n * c.q : 1e300;
     c.int32 = 9.6e-14 * n + 2.19e-13 * n * c.q;
     return c;
@@ -210,8 +210,8 @@ IntWalkCosts int_walk_costs(const int64_t* a, int n)
 // Where the sparse walk (columns in ascending_col_sort order) is the faster
 // one: from order 26 (below, a permanent is too small a job for its
 // per-chunk start) when the cost model above puts it below the dense exact
-// walk that applies (measured on 0/1 matrices with 4 entries per row: 1.9x
-// the FP64 walk at order 28, 4.1x at 32, 5.5x at 40, 5.2x on the config-3
+// walk that applies (measured on 0/1 matrices with 4 entries per row: 2.0x
+// the FP64 walk at order 28, 4.3x at 32, 6.7x at 40, 6.2x on the config-3
 // order-40 matrix; with larger entries 2.3x over the int32 walk on
 // order-30 curtis54 leaves).
 bool int_sparse_preferred(const int64_t* a, int n)
