@@ -522,7 +522,7 @@ extern "C" int spm_plan_codegen(const int64_t* a, int n, int options, int device
         return SPM_EHIP;
     }
     const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
-    gray_geometry(n, codegen_unroll_bits(n, true), waves, &p->L, &p->chunks, &p->tasks);
+    gray_geometry(n, codegen_int_unroll_bits(b.data(), n), waves, &p->L, &p->chunks, &p->tasks);
     const uint64_t need_blocks = (p->tasks + (kDenseBlock / kWaveLanes) - 1) / (kDenseBlock / kWaveLanes);
     p->grid = (int)std::min<uint64_t>(uint64_t(bpc) * cus, need_blocks);
     const double est_s = std::ldexp(1.0, n - 1) * 16.0 / 3.9e13;
@@ -621,7 +621,7 @@ extern "C" int spm_plan_codegen_real(const double* a, int n, int options, int pr
         return SPM_EHIP;
     }
     const int64_t waves = int64_t(bpc) * cus * (kDenseBlock / kWaveLanes);
-    gray_geometry(n, codegen_unroll_bits(n, false), waves, &p->L, &p->chunks, &p->tasks);
+    gray_geometry(n, codegen_real_unroll_bits(n), waves, &p->L, &p->chunks, &p->tasks);
     p->max_grid = bpc * cus;
     dense_grid(p);
     SPM_TRY(hipMalloc(&p->d_partial, sizeof(double) * p->rec * p->grid));
@@ -941,7 +941,11 @@ double est_gen_real(const Shape& s)
     const double pr = per_row(s);
     return 3e-5 + subsets(s.n) * (3.0e-14 * pr + 4e-15 * pr * pr);
 }
-double est_compile(const Shape& s) { return 0.1 + 2.5e-3 * std::ldexp(1.0, codegen_unroll_bits(s.n, true)) * s.n; }
+double est_compile(const Shape& s)
+{
+    const bool compact = s.walks.q < kGenCompactTermQ;  // terms formed in a loop: far less code per step
+    return 0.1 + (compact ? 4.6e-4 : 2.5e-3) * std::ldexp(1.0, gen_unroll_bits_int(s.n, compact)) * s.n;
+}
 double est_compile_real(const Shape& s) { return 0.15 + 1.2e-3 * double(s.nnz); }
 
 }  // namespace

@@ -109,11 +109,6 @@ std::vector<int64_t> apply_perm(const int64_t* a, int n, const std::vector<int>&
     return b;
 }
 
-// ---------------------------------------------------------------------------
-// Source emitter (integer entries).
-// ---------------------------------------------------------------------------
-namespace {
-
 // log2 of the steps per generated block.  A block's first flip goes
 // through a chain of bit tests on the block index, its 2^U - 1 others are
 // straight-line code with the column and direction known at compile time,
@@ -122,13 +117,23 @@ namespace {
 // patterns (config 3 / rows of 4 and 5 entries): U = 3 127.6 / 198.1 ms,
 // 0.9 s compile; U = 4 100.0 / 187.1 ms, 1.6 s; U = 5 77.1 / 175.8 ms,
 // 3.3 s.  Integer kernels: U = 3 up to order 30 (a permanent there takes
-// milliseconds, the compile dominates), 4 up to 45, 5 beyond (one walk takes
-// tens of seconds).  Real kernels form their term from cached suffix
-// products, so their code stays small: U = 5 from order 21 (order 40, N(0,1)
-// on 4 entries per row: 136 / 102 / 85 ms at U = 3 / 4 / 5, 0.25 s compile
-// each).
-int gen_unroll_bits_int(int n) { return n <= 30 ? 3 : (n <= 45 ? 4 : 5); }
+// milliseconds, the compile dominates), 4 up to 45, 5 beyond (one walk
+// takes tens of seconds).  Kernels whose terms are rarest (gen_compact_term)
+// form them in a loop and afford U = 5 from order 31 (config 3: 85 ms with
+// a 0.69 s compile against 100 ms and 1.58 s inlined at U = 4); with terms
+// merely uncommon the loop loses (rows of 4 and 5 entries at order 40, q
+// near 1e-4: 188 -> 611 ms).  Real kernels form their term from cached
+// suffix products, so their code stays small: U = 5 from order 21 (order
+// 40, N(0,1) on 4 entries per row: 136 / 102 / 85 ms at U = 3 / 4 / 5,
+// 0.25 s compile each).
+int gen_unroll_bits_int(int n, bool compact) { return n <= 30 ? 3 : ((compact || n > 45) ? 5 : 4); }
 int gen_unroll_bits_real(int n) { return n <= 20 ? 3 : 5; }
+
+// ---------------------------------------------------------------------------
+// Source emitter (integer entries).
+// ---------------------------------------------------------------------------
+namespace {
+
 
 // Waves per SIMD asked of the register allocator (0: the compiler's
 // choice).  Orders 29..40: the natural allocation (100-127 VGPRs) leaves 4;
@@ -188,15 +193,30 @@ void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& wb, const
 // test is a wave-wide ballot with a cold-branch hint: on sparse patterns
 // almost no wave holds a nonzero term, and a uniform, not-taken branch costs
 // far less per step than a lane-divergent region around the product.
-void emit_term(std::ostringstream& o, int n, bool negative, const char* ind)
+// COMPACT forms the product in a loop over the rows copied to a private
+// array instead of an inlined 128-bit chain: 2^U inlined chains make most
+// of a kernel's code and compile time (order 40 at U = 4: 15356 -> 3595
+// instructions), but the loop costs several times more when it runs, so it
+// is taken only where terms are rarest (gen_compact_term).
+void emit_term(std::ostringstream& o, int n, bool negative, bool compact, const char* ind)
 {
     o << ind << "if (__builtin_expect(__builtin_amdgcn_ballot_w64(Z == 0) != 0, 0)) {\n";
     o << ind << "  if (Z == 0) {\n";
-    o << ind << "    unsigned __int128 p = (unsigned __int128)(__int128)y0;\n";
-    int r = 1;
-    for (; r + 1 < n; r += 2)
-        o << ind << "    p *= (unsigned __int128)(__int128)((long long)y" << r << " * (long long)y" << r + 1 << ");\n";
-    if (r < n) o << ind << "    p *= (unsigned __int128)(__int128)y" << r << ";\n";
+    if (compact) {
+        o << ind << "    const int yy[" << n << "] = {";
+        for (int r = 0; r < n; ++r) o << (r ? ", " : "") << "y" << r;
+        o << "};\n";
+        o << ind << "    unsigned __int128 p = 1;\n";
+        o << ind << "#pragma nounroll\n";
+        o << ind << "    for (int r = 0; r < " << n << "; ++r) p *= (unsigned __int128)(__int128)yy[r];\n";
+    } else {
+        o << ind << "    unsigned __int128 p = (unsigned __int128)(__int128)y0;\n";
+        int r = 1;
+        for (; r + 1 < n; r += 2)
+            o << ind << "    p *= (unsigned __int128)(__int128)((long long)y" << r << " * (long long)y" << r + 1
+              << ");\n";
+        if (r < n) o << ind << "    p *= (unsigned __int128)(__int128)y" << r << ";\n";
+    }
     o << ind << "    acc " << (negative ? "-" : "+") << "= p;\n";
     o << ind << "  }\n";
     o << ind << "}\n";
@@ -206,7 +226,8 @@ void emit_term(std::ostringstream& o, int n, bool negative, const char* ind)
 
 std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::string& name)
 {
-    const int U = gen_unroll_bits_int(n);
+    const bool compact = gen_compact_term(b.data(), n);
+    const int U = gen_unroll_bits_int(n, compact);
     // y = 2x, halved on rows with even sums (int_row_weights): wb holds the
     // walk's increments w_r b_rc
     std::vector<unsigned char> w(n);
@@ -278,11 +299,11 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
                 emit_flip_const(o, wb, zr, n, c, 1, (std::string(ind) + "    ").c_str());
                 o << ind << "}\n";
             }
-            emit_term(o, n, (j & 1) != 0, ind);
+            emit_term(o, n, (j & 1) != 0, compact, ind);
         }
     };
     o << "        {\n            const unsigned m = 0;\n";
-    emit_term(o, n, false, "            ");
+    emit_term(o, n, false, compact, "            ");
     block_tail("            ");
     o << "        }\n";
     // block m first flips column U + ctz(m), in the block's direction
@@ -296,7 +317,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
         emit_flip_lane(o, wb, zr, n, c, "lmm", "                ");
     }
     if (U < n - 1) o << "            }\n";
-    emit_term(o, n, false, "            ");
+    emit_term(o, n, false, compact, "            ");
     block_tail("            ");
     o << "        }\n";
     o << "    }\n"
@@ -582,7 +603,9 @@ uint64_t real_pattern_hash(const double* b, int n, int options, int precision)
     return h;
 }
 
-int codegen_unroll_bits(int n, bool integral) { return integral ? gen_unroll_bits_int(n) : gen_unroll_bits_real(n); }
+int codegen_real_unroll_bits(int n) { return gen_unroll_bits_real(n); }
+bool gen_compact_term(const int64_t* b, int n) { return int_nonzero_term_fraction(b, n) < kGenCompactTermQ; }
+int codegen_int_unroll_bits(const int64_t* b, int n) { return gen_unroll_bits_int(n, gen_compact_term(b, n)); }
 
 // FNV-1a over the pattern, values and generator options.
 uint64_t pattern_hash(const int64_t* b, int n, int options)
@@ -596,7 +619,8 @@ uint64_t pattern_hash(const int64_t* b, int n, int options)
     };
     mix((uint64_t)n);
     mix((uint64_t)options);
-    mix((uint64_t)gen_unroll_bits_int(n));
+    mix((uint64_t)codegen_int_unroll_bits(b, n));
+    mix((uint64_t)gen_compact_term(b, n));
     for (int e = 0; e < n * n; ++e) mix((uint64_t)b[e]);
     return h;
 }

@@ -97,7 +97,15 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
 std::string emit_real_kernel(const std::vector<double>& b, int n, bool compensated, const std::string& name);
 int real_levels_count(const std::vector<double>& b, int n);
 uint64_t real_pattern_hash(const double* b, int n, int options, int precision);
-int codegen_unroll_bits(int n, bool integral);  // log2 of a generated kernel's block length
+// log2 of a generated kernel's block length: real kernels by order, integer
+// kernels by order and whether their terms are rare enough to be formed in
+// a loop (gen_compact_term: the fraction of terms with no zero row below
+// kGenCompactTermQ)
+constexpr double kGenCompactTermQ = 1e-7;
+int gen_unroll_bits_int(int n, bool compact);
+bool gen_compact_term(const int64_t* b, int n);
+int codegen_int_unroll_bits(const int64_t* b, int n);
+int codegen_real_unroll_bits(int n);
 uint64_t pattern_hash(const int64_t* b, int n, int options);
 int codegen_get_kernel(int device, uint64_t hash, const std::string& src, const std::string& name, hipFunction_t* fn,
                        double* compile_s, bool* cached);
