@@ -43,6 +43,34 @@ def test_signed_sparse_ints(n):
     assert api.int_exact(a).exact == ref
 
 
+def two_per_row(n, seed, signed):
+    """A seeded permutation plus one more entry per row; +-1 entries when
+    signed.  Half of a row's subset sums are zero, so a term is nonzero in
+    about 2^-n of the subsets."""
+    rng = np.random.default_rng(seed)
+    a = np.zeros((n, n), dtype=np.int64)
+    p = rng.permutation(n)
+    for i in range(n):
+        a[i, p[i]] = 1
+        a[i, rng.choice([j for j in range(n) if j != p[i]])] = 1
+    if signed:
+        a *= rng.choice([-1, 1], size=(n, n))
+    return a
+
+
+@pytest.mark.parametrize("n,seed,signed", [(24, 324, False), (24, 325, True), (26, 326, False), (26, 326, True),
+                                           (30, 330, False), (30, 334, True)])
+def test_rarest_terms_formed_in_a_loop(n, seed, signed):
+    # patterns whose nonzero-term fraction is below 1e-7 form their terms in
+    # a loop over the rows (8-step blocks at these orders, 32 from order 31):
+    # exact against the int128 oracle, signs included
+    a = two_per_row(n, seed, signed)
+    assert "const int yy[" in api.codegen_source(a)
+    ref, st = O.int_exact(a)
+    assert st == 0 and ref != 0
+    assert api.sparse_codegen(a).exact == ref
+
+
 def test_cache_reuses_compiled_kernel():
     a = W.sparse01(24, 99)
     with api.CodegenPlan(a) as p1:
