@@ -119,8 +119,9 @@ std::vector<int64_t> apply_perm(const int64_t* a, int n, const std::vector<int>&
 // 3.3 s.  Integer kernels: U = 3 up to order 30 (a permanent there takes
 // milliseconds, the compile dominates), 4 up to 45, 5 beyond (one walk
 // takes tens of seconds).  Kernels whose terms are rarest (gen_compact_term)
-// form them in a loop and afford U = 5 from order 31 (config 3: 85 ms with
-// a 0.69 s compile against 100 ms and 1.58 s inlined at U = 4); with terms
+// test them once per block and form them in a loop, and afford U = 5 from
+// order 31 (config 3: 71 ms with a 0.55 s compile against 100 ms and 1.58 s
+// inlined at U = 4); with terms
 // merely uncommon the loop loses (rows of 4 and 5 entries at order 40, q
 // near 1e-4: 188 -> 611 ms).  Real kernels form their term from cached
 // suffix products, so their code stays small: U = 5 from order 21 (order
@@ -197,9 +198,16 @@ void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& wb, const
 // array instead of an inlined 128-bit chain: 2^U inlined chains make most
 // of a kernel's code and compile time (order 40 at U = 4: 15356 -> 3595
 // instructions), but the loop costs several times more when it runs, so it
-// is taken only where terms are rarest (gen_compact_term).
-void emit_term(std::ostringstream& o, int n, bool negative, bool compact, const char* ind)
+// is taken only where terms are rarest (gen_compact_term).  FOLD (compact
+// kernels' steps) only folds Z into the block's least zero count Zm; SIGN_VAR
+// names a run-time sign (true: subtract).
+void emit_term(std::ostringstream& o, int n, bool negative, bool compact, const char* ind, bool fold = false,
+               bool first = false, const char* sign_var = nullptr)
 {
+    if (fold) {
+        o << ind << (first ? "Zm = Z;\n" : "Zm = min(Zm, Z);\n");
+        return;
+    }
     o << ind << "if (__builtin_expect(__builtin_amdgcn_ballot_w64(Z == 0) != 0, 0)) {\n";
     o << ind << "  if (Z == 0) {\n";
     if (compact) {
@@ -217,7 +225,35 @@ void emit_term(std::ostringstream& o, int n, bool negative, bool compact, const 
               << ");\n";
         if (r < n) o << ind << "    p *= (unsigned __int128)(__int128)y" << r << ";\n";
     }
-    o << ind << "    acc " << (negative ? "-" : "+") << "= p;\n";
+    if (sign_var) o << ind << "    acc = " << sign_var << " ? acc - p : acc + p;\n";
+    else o << ind << "    acc " << (negative ? "-" : "+") << "= p;\n";
+    o << ind << "  }\n";
+    o << ind << "}\n";
+}
+
+// Compact kernels test no term per step: each step folds Z into the block's
+// least zero count Zm (one VALU op, no branch waiting on a compare).  At the
+// end of block M, when some lane's Zm is 0, the wave walks the block back
+// from its last step (undoing flips 2^U - 1 .. 1 through flipc, a switch
+// over columns 0..U-1), forming each subset's term on the way, and then
+// forward again to restore its rows.
+void emit_block_replay(std::ostringstream& o, int U, const char* M, const char* ind)
+{
+    const unsigned last = (1u << U) - 1;
+    o << ind << "if (__builtin_expect(__builtin_amdgcn_ballot_w64(Zm == 0) != 0, 0)) {\n";
+    o << ind << "  termc(true);\n";
+    o << ind << "#pragma nounroll\n";
+    o << ind << "  for (unsigned j = " << last << "u; j >= 1u; --j) {\n";
+    o << ind << "    const unsigned c = __builtin_ctz(j);\n";
+    o << ind << "    const unsigned bit = c == " << U - 1 << "u ? (" << M << " & 1u) : ((j >> (c + 1)) & 1u);\n";
+    o << ind << "    flipc(c, ~(-(int)bit));\n";
+    o << ind << "    termc(((j - 1u) & 1u) != 0);\n";
+    o << ind << "  }\n";
+    o << ind << "#pragma nounroll\n";
+    o << ind << "  for (unsigned j = 1u; j <= " << last << "u; ++j) {\n";
+    o << ind << "    const unsigned c = __builtin_ctz(j);\n";
+    o << ind << "    const unsigned bit = c == " << U - 1 << "u ? (" << M << " & 1u) : ((j >> (c + 1)) & 1u);\n";
+    o << ind << "    flipc(c, -(int)bit);\n";
     o << ind << "  }\n";
     o << ind << "}\n";
 }
@@ -284,6 +320,22 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
     o << "        const int lm = -(int)(h & 1ull);  // direction of column L-1: -1 = remove\n";
     o << "        const unsigned nblk = 1u << (L - " << U << ");\n"
          "        const unsigned half = nblk >> 1;\n";
+    if (compact) {
+        o << "        int Zm;  // the block's least Z\n";
+        o << "        auto flipc = [&](unsigned c, int mk) __attribute__((always_inline)) {\n"
+             "            switch (c) {\n";
+        for (int c = 0; c < U; ++c) {
+            o << "            case " << c << "u:\n";
+            emit_flip_lane(o, wb, zr, n, c, "mk", "                ");
+            o << "                break;\n";
+        }
+        o << "            default: break;\n"
+             "            }\n"
+             "        };\n";
+        o << "        auto termc = [&](bool neg) __attribute__((always_inline)) {\n";
+        emit_term(o, n, false, true, "            ", false, false, "neg");
+        o << "        };\n";
+    }
     // steps j = 1 .. 2^U - 1 of block m: compile-time columns ctz(j); the
     // direction of column U-1 is bit 0 of m
     auto block_tail = [&](const char* ind) {
@@ -299,11 +351,12 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
                 emit_flip_const(o, wb, zr, n, c, 1, (std::string(ind) + "    ").c_str());
                 o << ind << "}\n";
             }
-            emit_term(o, n, (j & 1) != 0, compact, ind);
+            emit_term(o, n, (j & 1) != 0, compact, ind, compact);
         }
+        if (compact) emit_block_replay(o, U, "m", ind);
     };
     o << "        {\n            const unsigned m = 0;\n";
-    emit_term(o, n, false, compact, "            ");
+    emit_term(o, n, false, compact, "            ", compact, true);
     block_tail("            ");
     o << "        }\n";
     // block m first flips column U + ctz(m), in the block's direction
@@ -317,7 +370,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
         emit_flip_lane(o, wb, zr, n, c, "lmm", "                ");
     }
     if (U < n - 1) o << "            }\n";
-    emit_term(o, n, false, compact, "            ");
+    emit_term(o, n, false, compact, "            ", compact, true);
     block_tail("            ");
     o << "        }\n";
     o << "    }\n"
