@@ -922,11 +922,11 @@ double per_row(const Shape& s) { return double(s.nnz) / s.n; }
 // 1.6-2.3e-13 s, 4.9-6.7e-13, 1.1-1.6e-12 (as (entries per row)^2.75);
 // real 1.6-1.8e-13, 2.4-2.8e-13, 3.5-3.8e-13 (the quadratic term keeps
 // dense patterns on the dense walk).  Kernels whose terms are rarest (q
-// below kGenCompactTermQ: one term test per block) run 0.82x those integer
-// rates (order 40, 4 per row: 63-71 ms against 78-86 ms with a test per
-// step).  Compiles: first-call minus cached times; integer kernels about
-// 2.5e-3 2^U n s (U = log2 of the block length for order n), 3.5e-4 2^U n s
-// where terms are rarest, real ones 0.25-0.35 s.
+// below kGenCompactTermQ: one term test per block) run 0.7x those integer
+// rates (order 40: 4 per row 51-58 ms against 78-86 ms with a test per
+// step, 6 per row 223 against 272 ms).  Compiles: first-call minus cached
+// times; integer kernels about 2.5e-3 2^U n s (U = log2 of the block length
+// for order n), 2e-4 2^U n s where terms are rarest, real ones 0.25-0.35 s.
 double est_dense(const Shape& s) { return 3e-5 + subsets(s.n) * 2.0 * s.n / 3.5e13; }
 double est_int_walk(const Shape& s)
 {
@@ -937,7 +937,7 @@ double est_int_walk(const Shape& s)
 double est_gen_int(const Shape& s)
 {
     const double pr = per_row(s);
-    const double rare = s.walks.q < kGenCompactTermQ ? 0.82 : 1.0;
+    const double rare = s.walks.q < kGenCompactTermQ ? 0.7 : 1.0;
     return 3e-5 + subsets(s.n) * rare * (4.3e-15 * std::pow(pr, 2.75) + 2.3e-13 * s.n * s.walks.q);
 }
 double est_gen_real(const Shape& s)
@@ -947,8 +947,8 @@ double est_gen_real(const Shape& s)
 }
 double est_compile(const Shape& s)
 {
-    const bool compact = s.walks.q < kGenCompactTermQ;  // terms formed in a loop: far less code per step
-    return 0.1 + (compact ? 3.5e-4 : 2.5e-3) * std::ldexp(1.0, gen_unroll_bits_int(s.n, compact)) * s.n;
+    const bool compact = s.walks.q < kGenCompactTermQ;  // one term test per block: far less code
+    return 0.1 + (compact ? 2e-4 : 2.5e-3) * std::ldexp(1.0, gen_unroll_bits_int(s.n, compact)) * s.n;
 }
 double est_compile_real(const Shape& s) { return 0.15 + 1.2e-3 * double(s.nnz); }
 

@@ -119,11 +119,13 @@ std::vector<int64_t> apply_perm(const int64_t* a, int n, const std::vector<int>&
 // 3.3 s.  Integer kernels: U = 3 up to order 30 (a permanent there takes
 // milliseconds, the compile dominates), 4 up to 45, 5 beyond (one walk
 // takes tens of seconds).  Kernels whose terms are rarest (gen_compact_term)
-// test them once per block and form them in a loop, and afford U = 5 from
-// order 31 (config 3: 71 ms with a 0.55 s compile against 100 ms and 1.58 s
-// inlined at U = 4); with terms
-// merely uncommon the loop loses (rows of 4 and 5 entries at order 40, q
-// near 1e-4: 188 -> 611 ms).  Real kernels form their term from cached
+// test them once per block, with one copy of the term product, and afford
+// U = 5 from order 31 (config 3: 58 ms with a 0.31 s compile, against 86 ms
+// and 0.69 s testing each step with the product in a loop, 100 ms and
+// 1.58 s inlined at U = 4; rows of 6 entries at order 40: 272 -> 223 ms,
+// 1.72 -> 0.40 s).  With more terms the block replays cost more than they
+// save (rows of 8 entries, about 3e-4 of the subsets with no zero row:
+// 606 -> 1099 ms).  Real kernels form their term from cached
 // suffix products, so their code stays small: U = 5 from order 21 (order
 // 40, N(0,1) on 4 entries per row: 136 / 102 / 85 ms at U = 3 / 4 / 5,
 // 0.25 s compile each).
@@ -193,16 +195,12 @@ void emit_flip_lane(std::ostringstream& o, const std::vector<int64_t>& wb, const
 // The term of the current subset, on the lanes where no row is zero.  The
 // test is a wave-wide ballot with a cold-branch hint: on sparse patterns
 // almost no wave holds a nonzero term, and a uniform, not-taken branch costs
-// far less per step than a lane-divergent region around the product.
-// COMPACT forms the product in a loop over the rows copied to a private
-// array instead of an inlined 128-bit chain: 2^U inlined chains make most
-// of a kernel's code and compile time (order 40 at U = 4: 15356 -> 3595
-// instructions), but the loop costs several times more when it runs, so it
-// is taken only where terms are rarest (gen_compact_term).  FOLD (compact
-// kernels' steps) only folds Z into the block's least zero count Zm; SIGN_VAR
-// names a run-time sign (true: subtract).
-void emit_term(std::ostringstream& o, int n, bool negative, bool compact, const char* ind, bool fold = false,
-               bool first = false, const char* sign_var = nullptr)
+// far less per step than a lane-divergent region around the product, an
+// inlined 128-bit chain over row pairs.  FOLD (steps of kernels whose terms
+// are rarest, gen_compact_term) only folds Z into the block's least zero
+// count Zm; SIGN_VAR names a run-time sign (true: subtract).
+void emit_term(std::ostringstream& o, int n, bool negative, const char* ind, bool fold = false, bool first = false,
+               const char* sign_var = nullptr)
 {
     if (fold) {
         o << ind << (first ? "Zm = Z;\n" : "Zm = min(Zm, Z);\n");
@@ -210,21 +208,11 @@ void emit_term(std::ostringstream& o, int n, bool negative, bool compact, const 
     }
     o << ind << "if (__builtin_expect(__builtin_amdgcn_ballot_w64(Z == 0) != 0, 0)) {\n";
     o << ind << "  if (Z == 0) {\n";
-    if (compact) {
-        o << ind << "    const int yy[" << n << "] = {";
-        for (int r = 0; r < n; ++r) o << (r ? ", " : "") << "y" << r;
-        o << "};\n";
-        o << ind << "    unsigned __int128 p = 1;\n";
-        o << ind << "#pragma nounroll\n";
-        o << ind << "    for (int r = 0; r < " << n << "; ++r) p *= (unsigned __int128)(__int128)yy[r];\n";
-    } else {
-        o << ind << "    unsigned __int128 p = (unsigned __int128)(__int128)y0;\n";
-        int r = 1;
-        for (; r + 1 < n; r += 2)
-            o << ind << "    p *= (unsigned __int128)(__int128)((long long)y" << r << " * (long long)y" << r + 1
-              << ");\n";
-        if (r < n) o << ind << "    p *= (unsigned __int128)(__int128)y" << r << ";\n";
-    }
+    o << ind << "    unsigned __int128 p = (unsigned __int128)(__int128)y0;\n";
+    int r = 1;
+    for (; r + 1 < n; r += 2)
+        o << ind << "    p *= (unsigned __int128)(__int128)((long long)y" << r << " * (long long)y" << r + 1 << ");\n";
+    if (r < n) o << ind << "    p *= (unsigned __int128)(__int128)y" << r << ";\n";
     if (sign_var) o << ind << "    acc = " << sign_var << " ? acc - p : acc + p;\n";
     else o << ind << "    acc " << (negative ? "-" : "+") << "= p;\n";
     o << ind << "  }\n";
@@ -241,13 +229,13 @@ void emit_block_replay(std::ostringstream& o, int U, const char* M, const char* 
 {
     const unsigned last = (1u << U) - 1;
     o << ind << "if (__builtin_expect(__builtin_amdgcn_ballot_w64(Zm == 0) != 0, 0)) {\n";
-    o << ind << "  termc(true);\n";
     o << ind << "#pragma nounroll\n";
-    o << ind << "  for (unsigned j = " << last << "u; j >= 1u; --j) {\n";
+    o << ind << "  for (unsigned j = " << last << "u;; --j) {\n";
+    o << ind << "    termc((j & 1u) != 0);\n";
+    o << ind << "    if (j == 0u) break;\n";
     o << ind << "    const unsigned c = __builtin_ctz(j);\n";
     o << ind << "    const unsigned bit = c == " << U - 1 << "u ? (" << M << " & 1u) : ((j >> (c + 1)) & 1u);\n";
     o << ind << "    flipc(c, ~(-(int)bit));\n";
-    o << ind << "    termc(((j - 1u) & 1u) != 0);\n";
     o << ind << "  }\n";
     o << ind << "#pragma nounroll\n";
     o << ind << "  for (unsigned j = 1u; j <= " << last << "u; ++j) {\n";
@@ -333,7 +321,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
              "            }\n"
              "        };\n";
         o << "        auto termc = [&](bool neg) __attribute__((always_inline)) {\n";
-        emit_term(o, n, false, true, "            ", false, false, "neg");
+        emit_term(o, n, false, "            ", false, false, "neg");
         o << "        };\n";
     }
     // steps j = 1 .. 2^U - 1 of block m: compile-time columns ctz(j); the
@@ -351,12 +339,12 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
                 emit_flip_const(o, wb, zr, n, c, 1, (std::string(ind) + "    ").c_str());
                 o << ind << "}\n";
             }
-            emit_term(o, n, (j & 1) != 0, compact, ind, compact);
+            emit_term(o, n, (j & 1) != 0, ind, compact);
         }
         if (compact) emit_block_replay(o, U, "m", ind);
     };
     o << "        {\n            const unsigned m = 0;\n";
-    emit_term(o, n, false, compact, "            ", compact, true);
+    emit_term(o, n, false, "            ", compact, true);
     block_tail("            ");
     o << "        }\n";
     // block m first flips column U + ctz(m), in the block's direction
@@ -370,7 +358,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
         emit_flip_lane(o, wb, zr, n, c, "lmm", "                ");
     }
     if (U < n - 1) o << "            }\n";
-    emit_term(o, n, false, compact, "            ", compact, true);
+    emit_term(o, n, false, "            ", compact, true);
     block_tail("            ");
     o << "        }\n";
     o << "    }\n"

@@ -60,12 +60,13 @@ def two_per_row(n, seed, signed):
 
 @pytest.mark.parametrize("n,seed,signed", [(24, 324, False), (24, 325, True), (26, 326, False), (26, 326, True),
                                            (30, 330, False), (30, 334, True)])
-def test_rarest_terms_formed_in_a_loop(n, seed, signed):
-    # patterns whose nonzero-term fraction is below 1e-7 form their terms in
-    # a loop over the rows (8-step blocks at these orders, 32 from order 31):
-    # exact against the int128 oracle, signs included
+def test_rarest_terms_tested_per_block(n, seed, signed):
+    # patterns whose estimated nonzero-term fraction is below 5e-7 test their
+    # terms once per block and form them on a walk back through the block
+    # (8-step blocks at these orders, 32 from order 31): exact against the
+    # int128 oracle, signs included
     a = two_per_row(n, seed, signed)
-    assert "const int yy[" in api.codegen_source(a)
+    assert "Zm = min(Zm, Z);" in api.codegen_source(a)
     ref, st = O.int_exact(a)
     assert st == 0 and ref != 0
     assert api.sparse_codegen(a).exact == ref
