@@ -33,8 +33,8 @@ TABLE = [
     (16, 4, "int01", "int"),
     (16, 4, "real", "dense"),
     (36, 4, "real", "dense"),           # 6 ms generated + 0.24 s compile against 71 ms dense
-    # order 40, 4 per row: the compressed-column walk (0.25 s) beats a
-    # 1.6 s compile + 0.09 s generated kernel; real entries: the generated
+    # order 40, 4 per row: the compressed-column walk (0.23 s) beats a
+    # 0.32 s compile + 0.05 s generated kernel; real entries: the generated
     # FP64 kernel (0.09 s + 0.25 s compile) beats the dense walk (1.26 s)
     (40, 4, "int01", "int"),
     (40, 4, "real", "gen_real"),

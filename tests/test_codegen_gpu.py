@@ -132,9 +132,10 @@ def test_disk_cache_across_processes(tmp_path):
 
 
 def test_auto_choice_counts_a_compiled_kernel_as_a_load(tmp_path, monkeypatch):
-    # order 40, 4 entries per row: a compile (1.5 s) does not pay for one
+    # order 40, 4 entries per row: a compile (0.3 s) does not pay for one
     # permanent, but once the pattern's kernel is compiled the automatic
-    # path takes it (0.13 s against 0.35 s for the compressed-column walk);
+    # path takes it (about 0.05 s against 0.23 s for the compressed-column
+    # walk);
     # spm_choose_path, the choice for kernels not yet compiled, stays put.
     # An empty on-disk cache of its own, and a pattern no other test uses.
     monkeypatch.setenv("SPM_KERNEL_CACHE", str(tmp_path))
