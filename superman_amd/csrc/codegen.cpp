@@ -143,11 +143,13 @@ namespace {
 // at 96 VGPRs the compiler spills only around the term product and the
 // final reduction (config-3 order-40 pattern: 155 -> 141 ms).  Orders
 // 41..48: 129-130 VGPRs leave 3; 128 spill at most 8 bytes (order-42
-// pattern: 706 -> 590 ms).  Orders up to 28 reach 5 without it.
-int gen_waves_per_simd(int n)
+// pattern: 706 -> 590 ms); kernels testing their terms once per block
+// (RARE) hold one product and take 5 there too (order 44, 6 entries per
+// row: 2158 -> 1892 ms; no gain at 6).  Orders up to 28 reach 5 without it.
+int gen_waves_per_simd(int n, bool rare)
 {
     if (n >= 29 && n <= 40) return 5;
-    if (n >= 41 && n <= 48) return 4;
+    if (n >= 41 && n <= 48) return rare ? 5 : 4;
     return 0;
 }
 
@@ -273,7 +275,7 @@ std::string emit_int_kernel(const std::vector<int64_t>& b, int n, const std::str
     std::ostringstream o;
     o << "// generated: SpaRyser kernel for one " << n << "x" << n << " integer pattern\n";
     o << "extern \"C\" __global__ void __launch_bounds__(256)";
-    if (const int w = gen_waves_per_simd(n)) o << " __attribute__((amdgpu_waves_per_eu(" << w << ")))";
+    if (const int w = gen_waves_per_simd(n, compact)) o << " __attribute__((amdgpu_waves_per_eu(" << w << ")))";
     o << "\n";
     o << name << "(unsigned long long* __restrict__ partial, unsigned long long task_begin,\n"
       << "    unsigned long long task_end, unsigned long long num_chunks, int L, int accumulate)\n{\n";
