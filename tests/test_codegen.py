@@ -58,6 +58,32 @@ def test_generated_source_compiles(n, seed):
     assert info["code_bytes"] > 1000
 
 
+def rows_of(n, per_row, seed):
+    """0/1, a seeded permutation plus per_row - 1 further entries per row."""
+    rng = np.random.default_rng(seed)
+    a = np.zeros((n, n), dtype=np.int64)
+    perm = rng.permutation(n)
+    for i in range(n):
+        a[i, perm[i]] = 1
+        others = rng.choice([j for j in range(n) if j != perm[i]], size=per_row - 1, replace=False)
+        a[i, others] = 1
+    return a
+
+
+@pytest.mark.parametrize("n,per_row,per_block,waves", [(40, 4, True, 5), (40, 6, True, 5), (40, 8, False, 5),
+                                                       (36, 6, False, 5), (44, 6, True, 5), (44, 8, False, 4)])
+def test_term_test_per_block_or_per_step(n, per_row, per_block, waves):
+    # the estimated nonzero-term fraction (row-independence estimate; rows of
+    # 4 / 6 / 8 entries at order 40: 6.8e-9 / 3.1e-7 / 2.8e-6) picks one term
+    # test per 32-step block below 5e-7, else one per step; kernels testing
+    # per block ask for 5 waves per SIMD up to order 48, the others for 4
+    # past order 40
+    src = api.codegen_source(rows_of(n, per_row, 7000 * n + per_row))
+    assert ("Zm = min(Zm, Z);" in src) == per_block
+    assert ("ballot_w64(Zm == 0)" in src) == per_block
+    assert "amdgpu_waves_per_eu(%d)" % waves in src
+
+
 def test_codegen_rejects_non_integers():
     with pytest.raises(ValueError):
         api.codegen_source(np.full((6, 6), 0.5))
